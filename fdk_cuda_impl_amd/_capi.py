@@ -1,0 +1,162 @@
+"""ctypes binding of ``libyk_hip.so`` (the C ABI declared in include/yk_hip.h).
+
+The library is loaded from this package directory.  Nothing here falls back to
+another implementation: a missing or unloadable library raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libyk_hip.so")
+
+# yk_status (mirrors YK::EApiErrorCode)
+OK, INVALID_CONFIG, INVALID_GEOMETRY, UNSUPPORTED, ALGORITHM_UNDER_TEST, INVALID_BUFFER, \
+    STREAMING_STATE, DEVICE_ERROR = range(8)
+STATUS_NAMES = {0: "None", 1: "InvalidConfig", 2: "InvalidGeometry", 3: "UnsupportedCombination",
+                4: "AlgorithmUnderTest", 5: "InvalidBuffer", 6: "StreamingStateError",
+                7: "DeviceError"}
+
+HOST, DEVICE = 0, 1
+F32, F16, BF16 = 0, 1, 2
+FP_JOSEPH, FP_SIDDON = 1, 2
+BP_JOSEPH_ADJOINT, BP_SIDDON_ADJOINT, BP_FDK, BP_FDK_PSEUDO_MATCHED = 8, 4, 6, 7
+PARKER_AUTO, PARKER_DISABLED, PARKER_ENABLED = 0, 1, 2
+FILTERS = {"none": 0, "ramlak": 1, "shepplogan": 2, "cosine": 3, "hann": 4, "hamming": 5,
+           "blackman": 6, "butterworth": 7, "kaiser": 8, "tukey": 9}
+STORAGE = {"f32": F32, "float32": F32, "f16": F16, "float16": F16, "bf16": BF16, "bfloat16": BF16}
+
+
+class YkError(RuntimeError):
+    """A yk_status other than YK_OK, with the library's message."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class View(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_float * 4), ("detS", ctypes.c_float * 4),
+                ("detU", ctypes.c_float * 4), ("detV", ctypes.c_float * 4),
+                ("angle", ctypes.c_float * 4)]
+
+
+class CircularScan(ctypes.Structure):
+    _fields_ = [("total_views", ctypes.c_int), ("views_per_turn", ctypes.c_int),
+                ("start_angle_rad", ctypes.c_float), ("rotation_direction", ctypes.c_int),
+                ("sid_mm", ctypes.c_float), ("sdd_mm", ctypes.c_float), ("z_mm", ctypes.c_float),
+                ("source_offset_mm", ctypes.c_float * 3)]
+
+
+class DetectorPose(ctypes.Structure):
+    _fields_ = [("offset_unv_mm", ctypes.c_float * 3), ("tilt_u_rad", ctypes.c_float),
+                ("tilt_v_rad", ctypes.c_float), ("tilt_n_rad", ctypes.c_float)]
+
+
+class FlatDetector(ctypes.Structure):
+    _fields_ = [("channels", ctypes.c_int), ("rows", ctypes.c_int),
+                ("channel_size_mm", ctypes.c_float), ("row_size_mm", ctypes.c_float),
+                ("pose", DetectorPose)]
+
+
+class VolumeGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("voxel_x_mm", ctypes.c_float), ("voxel_y_mm", ctypes.c_float),
+                ("voxel_z_mm", ctypes.c_float), ("center_mm", ctypes.c_float * 3)]
+
+
+class System(ctypes.Structure):
+    _fields_ = [("scan", CircularScan), ("detector", FlatDetector), ("volume", VolumeGrid)]
+
+
+class Filter(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("cutoff", ctypes.c_float), ("gain", ctypes.c_float),
+                ("butterworth_order", ctypes.c_float), ("kaiser_beta", ctypes.c_float),
+                ("tukey_alpha", ctypes.c_float)]
+
+
+class FdkDesc(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("filter", Filter), ("parker_mode", ctypes.c_int),
+                ("scan_range_rad", ctypes.c_float), ("start_angle_rad", ctypes.c_float),
+                ("rotation_direction", ctypes.c_int), ("storage", ctypes.c_int),
+                ("kahan", ctypes.c_int), ("batch_views", ctypes.c_int)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+_FP = ctypes.POINTER(ctypes.c_float)
+_DP = ctypes.POINTER(ctypes.c_double)
+_VP = ctypes.POINTER(View)
+
+SIGNATURES = {
+    "yk_abi_version": (_I, []),
+    "yk_last_error": (ctypes.c_char_p, []),
+    "yk_ctx_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "yk_ctx_destroy": (_I, [_P]),
+    "yk_ctx_set_stream": (_I, [_P, _P]),
+    "yk_ctx_reset_stream": (_I, [_P]),
+    "yk_ctx_stream": (_P, [_P]),
+    "yk_ctx_set_synchronous": (_I, [_P, _I]),
+    "yk_ctx_synchronize": (_I, [_P]),
+    "yk_geometry_build": (_I, [ctypes.POINTER(System), _VP, _I, ctypes.POINTER(_I)]),
+    "yk_geometry_derive": (_I, [_VP, _I, _I, _I, _F, _FP]),
+    "yk_geometry_fdk_coefficients": (_I, [_VP, _I, _I, _I, _F, _FP]),
+    "yk_filter_weights": (_I, [ctypes.POINTER(Filter), _I, _F, _FP]),
+    "yk_filter_padded_n": (_I, [_I]),
+    "yk_fdk_create": (_I, [_P, ctypes.POINTER(FdkDesc), _VP, _I, _I, _I, ctypes.POINTER(VolumeGrid),
+                           ctypes.POINTER(_P)]),
+    "yk_fdk_create_system": (_I, [_P, ctypes.POINTER(FdkDesc), ctypes.POINTER(System),
+                                  ctypes.POINTER(_P)]),
+    "yk_fdk_destroy": (_I, [_P]),
+    "yk_fdk_execute": (_I, [_P, _P, _I, _SZ, _I, _I, _P, _I, _SZ, _I]),
+    "yk_fdk_reset": (_I, [_P]),
+    "yk_fdk_received_views": (_I, [_P]),
+    "yk_fdk_complete": (_I, [_P]),
+    "yk_fdk_filter": (_I, [_P, _P, _I, _I, _P, _I, _P]),
+    "yk_fdk_backproject": (_I, [_P, _P, _I, _P, _I, _I, _P, _I]),
+    "yk_fdk_padded_n": (_I, [_P]),
+    "yk_fdk_get_weights": (_I, [_P, _FP]),
+    "yk_fdk_get_derived": (_I, [_P, _FP]),
+    "yk_fdk_get_coefficients": (_I, [_P, _FP]),
+    "yk_projector_create": (_I, [_P, _VP, _I, _I, _I, ctypes.POINTER(VolumeGrid), _F,
+                                 ctypes.POINTER(_P)]),
+    "yk_projector_destroy": (_I, [_P]),
+    "yk_project": (_I, [_P, _I, _P, _P, _I, _I, _I]),
+    "yk_backproject": (_I, [_P, _I, _P, _P, _I, _I, _I]),
+    "yk_cgls": (_I, [_P, _I, _I, _P, _P, _I, _DP]),
+    "yk_sart": (_I, [_P, _I, _I, _P, _P, _I, _F, _I]),
+    "yk_tv_gradient": (_I, [_P, ctypes.POINTER(VolumeGrid), _P, _F, _P, _DP]),
+    "yk_dot": (_I, [_P, _P, _P, _SZ, _DP]),
+    "yk_phantom_ellipsoids": (_I, [_P, _FP, _I, ctypes.POINTER(VolumeGrid), _I, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libyk_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run fdk_cuda_impl_amd.build.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != OK:
+        msg = lib().yk_last_error()
+        raise YkError(status, msg.decode() if msg else "")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))

@@ -1,0 +1,311 @@
+// FDK plan: geometry precompute (FP64), device records, the per-(block, view)
+// footprint table, workspaces and the streaming execute contract of the
+// reference FdkPipeline / Session FDK branch.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fdk.hpp"
+
+namespace yk {
+namespace {
+
+constexpr double kTwoPi = 6.28318530717958647692;
+
+size_t storage_bytes(int storage)
+{
+    switch (storage) {
+    case YK_F32: return 4;
+    case YK_F16:
+    case YK_BF16: return 2;
+    default: fail(YK_ERR_INVALID_CONFIG, "unknown storage type");
+    }
+}
+
+// Parker weight table w(view, u) * range / pi (parker_weight_kernel /
+// pk_uploadAngles, .../FDK/kernels/YkFDKParkerKernels.cu:46-145)
+void parker_table(const std::vector<yk_view>& views, const std::vector<DerivedD>& derived,
+    int nu, int nv, double start, int dir, double range, std::vector<float>& out)
+{
+    const double pi = kTwoPi / 2.0;
+    const double fscale = range / pi;
+    out.assign(views.size() * size_t(nu), 0.f);
+    for (size_t i = 0; i < views.size(); ++i) {
+        const ViewD g = widen(views[i]);
+        const double rx = derived[i].radial[0], ry = derived[i].radial[1];
+        const double cv = 0.5 * (nv - 1);
+        auto fan = [&](double u) {
+            const double x = g.detS[0] + u * g.detU[0] + cv * g.detV[0] - g.src[0];
+            const double y = g.detS[1] + u * g.detU[1] + cv * g.detV[1] - g.src[1];
+            return std::atan2(x * ry - y * rx, x * rx + y * ry);
+        };
+        double beta = dir * (g.angle - start);
+        while (beta < 0.0) beta += kTwoPi;
+        while (beta >= kTwoPi) beta -= kTwoPi;
+        const double Gamma = std::max(std::fabs(fan(0.0)), std::fabs(fan(double(nu - 1))));
+        for (int u = 0; u < nu; ++u) {
+            const double gamma = dir * fan(double(u));
+            const double t1 = 2.0 * (Gamma + gamma), t2 = pi + 2.0 * gamma, t3 = pi + 2.0 * Gamma;
+            double w;
+            if (beta <= 0.0) w = 0.0;
+            else if (beta < t1) {
+                const double d = Gamma + gamma;
+                const double s = std::sin(0.25 * pi * beta / (std::fabs(d) > 1e-6 ? d : 1e-6));
+                w = s * s;
+            } else if (beta <= t2) w = 1.0;
+            else if (beta < t3) {
+                const double d = Gamma - gamma;
+                const double s = std::sin(0.25 * pi * (t3 - beta) / (std::fabs(d) > 1e-6 ? d : 1e-6));
+                w = s * s;
+            } else w = 0.0;
+            out[i * nu + u] = float(w * fscale);
+        }
+    }
+}
+
+// validateParkerCoverage_ (YkFdkPipeline.hpp:597-635): range >= pi + 2 max fan
+void check_parker_coverage(const std::vector<yk_view>& views, int nu, int nv, double range)
+{
+    if (range >= kTwoPi - 1e-5) fail(YK_ERR_INVALID_CONFIG, "Parker weighting needs a short scan (< 2 pi)");
+    double max_fan = 0.0;
+    for (const yk_view& v : views) {
+        const ViewD g = widen(v);
+        const double cu = 0.5 * (nu - 1), cv = 0.5 * (nv - 1);
+        const double px = g.detS[0] + cu * g.detU[0] + cv * g.detV[0] - g.src[0];
+        const double py = g.detS[1] + cu * g.detU[1] + cv * g.detV[1] - g.src[1];
+        for (double u : { 0.0, double(nu - 1) }) {
+            const double x = g.detS[0] + u * g.detU[0] + cv * g.detV[0] - g.src[0];
+            const double y = g.detS[1] + u * g.detU[1] + cv * g.detV[1] - g.src[1];
+            max_fan = std::max(max_fan, std::fabs(std::atan2(x * py - y * px, x * px + y * py)));
+        }
+    }
+    if (range + 1e-5 < kTwoPi / 2.0 + 2.0 * max_fan)
+        fail(YK_ERR_INVALID_CONFIG, "short-scan range below pi + 2 * fan angle");
+}
+
+}  // namespace
+}  // namespace yk
+
+using namespace yk;
+
+yk_fdk::yk_fdk(yk_ctx* c, const yk_fdk_desc& d, const yk_view* v, int n, int nu_, int nv_,
+    const yk_volume_grid& g)
+    : ctx(c), desc(d), nu(nu_), nv(nv_), n_views(n), grid(g)
+{
+    if (!v || n <= 0) fail(YK_ERR_INVALID_GEOMETRY, "FDK needs at least one view");
+    if (nu <= 0 || nv <= 0) fail(YK_ERR_INVALID_GEOMETRY, "invalid detector size");
+    validate_volume(grid);
+    validate_filter(desc.filter);
+    storage_bytes(desc.storage);
+    if (desc.rotation_direction != 1 && desc.rotation_direction != -1)
+        fail(YK_ERR_INVALID_CONFIG, "rotation direction must be +1 or -1");
+    if (desc.parker_mode < YK_PARKER_AUTO || desc.parker_mode > YK_PARKER_ENABLED)
+        fail(YK_ERR_INVALID_CONFIG, "invalid Parker mode");
+    scan_range = desc.scan_range_rad > 0.f ? double(desc.scan_range_rad) : kTwoPi;
+    if (!std::isfinite(scan_range)) fail(YK_ERR_INVALID_CONFIG, "scan range must be finite");
+    views.assign(v, v + n);
+    vol = volume_geom(grid);
+
+    ctx->activate();
+    hipStream_t s = ctx->stream;
+    derive_geometry(views, nu, nv, scan_range, derived);
+    fdk_coefficients(views, derived, coeffs);
+
+    // resolveParkerEnabled (include/YKCBCT/interface/YkSystemReconstruction.hpp:175-185)
+    parker = desc.parker_mode == YK_PARKER_ENABLED ||
+        (desc.parker_mode == YK_PARKER_AUTO && scan_range < kTwoPi - 1e-5);
+    if (parker) check_parker_coverage(views, nu, nv, scan_range);
+
+    // filter: weights use du of the first view (YkFDKFilterProcessor.hpp:111)
+    n_pad = padded_n(nu);
+    filter_weights(desc.filter, n_pad, derived[0].du, weights);
+    d_weights.upload(weights.data(), weights.size(), s);
+    std::vector<float2> tw(n_pad);
+    for (int k = 0; k < n_pad; ++k) {
+        const double a = -kTwoPi * double(k) / double(n_pad);
+        tw[k] = make_float2(float(std::cos(a)), float(std::sin(a)));
+    }
+    d_twiddles.upload(tw.data(), tw.size(), s);
+
+    std::vector<FilterView> fv(n);
+    std::vector<BpCoeff> bc(n);
+    std::vector<double> c64(size_t(n) * 12);
+    general = false;
+    for (int i = 0; i < n; ++i) {
+        FilterView& f = fv[i];
+        std::memset(&f, 0, sizeof(f));
+        std::memcpy(f.src, views[i].src, sizeof(f.src));
+        std::memcpy(f.detS, views[i].detS, sizeof(f.detS));
+        std::memcpy(f.detU, views[i].detU, sizeof(f.detU));
+        std::memcpy(f.detV, views[i].detV, sizeof(f.detV));
+        f.sdd_radial = float(derived[i].sdd_radial);
+        f.wgeom = float(derived[i].sdd_radial / derived[i].sod / 2.0);
+        f.start_u = start_u(nu, n_pad, derived[i].off_u);
+        BpCoeff& b = bc[i];
+        std::memset(&b, 0, sizeof(b));
+        for (int k = 0; k < 4; ++k) {
+            b.cu[k] = float(coeffs[i].cu[k]);
+            b.cv[k] = float(coeffs[i].cv[k]);
+            b.cd[k] = float(coeffs[i].cd[k]);
+            b.cr[k] = float(coeffs[i].cr[k]);
+            c64[size_t(i) * 12 + k] = coeffs[i].cu[k];
+            c64[size_t(i) * 12 + 4 + k] = coeffs[i].cv[k];
+            c64[size_t(i) * 12 + 8 + k] = coeffs[i].cd[k];
+        }
+        b.wbase = float(coeffs[i].wbase);
+        if (b.cu[2] != 0.f || b.cd[2] != 0.f) general = true;
+        if (b.cr[2] != 0.f) fail(YK_ERR_INVALID_GEOMETRY, "radial ray with a z component");
+    }
+    d_fviews.upload(fv.data(), fv.size(), s);
+    d_coeffs.upload(bc.data(), bc.size(), s);
+    d_c64.upload(c64.data(), c64.size(), s);
+    if (parker) {
+        std::vector<float> pt;
+        parker_table(views, derived, nu, nv, desc.start_angle_rad, desc.rotation_direction,
+            scan_range, pt);
+        d_parker.upload(pt.data(), pt.size(), s);
+    }
+    build_tiles();
+    YK_HIP(hipStreamSynchronize(s));
+
+    batch = desc.batch_views > 0 ? std::min(desc.batch_views, n) : n;
+    // keep the filtered workspace below 16 GiB by default
+    const size_t per_view = size_t(nu) * nv * 4;
+    if (desc.batch_views <= 0) {
+        const size_t cap = size_t(16) << 30;
+        batch = int(std::max<size_t>(1, std::min<size_t>(size_t(n), cap / per_view)));
+    }
+}
+
+void yk_fdk::build_tiles()
+{
+    hipStream_t s = ctx->stream;
+    const size_t max_lds = 96 * 1024;
+    const size_t quad = desc.storage == YK_F32 ? 16 : 8;
+    for (bz = bp_block_z(general ? 1 : 0); bz >= 8; bz /= 2) {
+        nbx = (grid.nx + 15) / 16;
+        nby = (grid.ny + 15) / 16;
+        nbz = (grid.nz + bz - 1) / bz;
+        const size_t entries = size_t(nbx) * nby * nbz * n_views;
+        d_tiles.resize(entries);
+        d_maxes.resize(8);
+        TileLaunch t{};
+        t.coeff64 = d_c64.get();
+        t.n_views = n_views;
+        t.vol = vol;
+        t.bx = 16; t.by = 16; t.bz = bz;
+        t.nbx = nbx; t.nby = nby; t.nbz = nbz;
+        t.nu = nu; t.nv = nv;
+        t.tiles = d_tiles.get();
+        t.max_quads = d_maxes.get();
+        launch_bp_tiles(t, s);
+        int maxes[4] = { 0, 0, 0, 0 };
+        YK_HIP(hipMemcpyAsync(maxes, d_maxes.get(), sizeof(maxes), hipMemcpyDeviceToHost, s));
+        YK_HIP(hipStreamSynchronize(s));
+        if (maxes[3]) fail(YK_ERR_INVALID_GEOMETRY, "volume reaches the source plane of some view");
+        lds_quads = std::max(1, maxes[2]);
+        if (size_t(lds_quads) * quad <= max_lds) return;
+    }
+    fail(YK_ERR_UNSUPPORTED, "detector footprint of a voxel block exceeds the LDS budget");
+}
+
+void yk_fdk::filter(const float* d_proj, int first, int count, void* d_out, int storage,
+    float* d_scales)
+{
+    if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
+    if (!d_proj || !d_out) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
+    storage_bytes(storage);
+    if (storage != YK_F32 && !d_scales) fail(YK_ERR_INVALID_BUFFER, "half / bfloat16 storage needs row scales");
+    FilterLaunch p{};
+    p.proj = d_proj;
+    p.out = d_out;
+    p.row_scales = d_scales;
+    p.views = d_fviews.get() + first;
+    p.parker = parker ? d_parker.get() + size_t(first) * nu : nullptr;
+    p.weights = d_weights.get();
+    p.twiddles = d_twiddles.get();
+    p.nu = nu; p.nv = nv; p.count = count; p.n = n_pad;
+    p.storage = storage;
+    launch_fdk_filter(p, ctx->stream);
+}
+
+void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_scales, int first,
+    int count, float* d_volume, int accumulate)
+{
+    if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
+    if (!d_filtered || !d_volume) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
+    storage_bytes(storage);
+    if (storage != YK_F32 && !d_scales) fail(YK_ERR_INVALID_BUFFER, "half / bfloat16 storage needs row scales");
+    const size_t quad = storage == YK_F32 ? 16 : 8;
+    if (size_t(lds_quads) * quad > 160 * 1024) fail(YK_ERR_UNSUPPORTED, "footprint exceeds LDS");
+    hipStream_t s = ctx->stream;
+    if (storage != YK_F32) launch_max_scale(d_scales, count * nv, d_maxes.get() + 4, s);
+    BpLaunch p{};
+    p.proj = d_filtered;
+    p.row_scales = storage == YK_F32 ? nullptr : d_scales;
+    p.gscale_bits = d_maxes.get() + 4;
+    p.coeffs = d_coeffs.get() + first;
+    p.tiles = d_tiles.get();
+    p.tile_view_stride = n_views;
+    p.tile_view_offset = first;
+    p.volume = d_volume;
+    p.vol = vol;
+    p.nu = nu; p.nv = nv; p.count = count;
+    p.storage = storage;
+    p.accumulate = accumulate;
+    p.kahan = desc.kahan;
+    p.general = general ? 1 : 0;
+    p.nbx = nbx; p.nby = nby; p.nbz = nbz;
+    p.lds_quads = lds_quads;
+    p.bz = bz;
+    launch_fdk_bp(p, s);
+}
+
+void yk_fdk::execute(const float* projection, int ploc, size_t pcap, int offset, int count,
+    float* volume, int vloc, size_t vcap, int clear_output)
+{
+    // YkReconstructionApi.cpp:353-370 and YkSession.cpp:693-728
+    if (!projection || !volume) fail(YK_ERR_INVALID_BUFFER, "projection and volume are required");
+    if ((ploc != YK_HOST && ploc != YK_DEVICE) || (vloc != YK_HOST && vloc != YK_DEVICE))
+        fail(YK_ERR_INVALID_BUFFER, "invalid memory location");
+    if (count <= 0) fail(YK_ERR_INVALID_CONFIG, "view_count must be positive");
+    if (offset != received) fail(YK_ERR_STREAMING_STATE, "view_offset must equal the views received so far");
+    if (offset + count > n_views) fail(YK_ERR_STREAMING_STATE, "batch runs past the prepared view sequence");
+    if (clear_output && received != 0) fail(YK_ERR_STREAMING_STATE, "clear_output is only valid for the first batch");
+    const size_t view_elems = size_t(nu) * nv;
+    const size_t vol_elems = size_t(grid.nx) * grid.ny * grid.nz;
+    if (pcap < view_elems * count) fail(YK_ERR_INVALID_BUFFER, "projection buffer smaller than view_count views");
+    if (vcap < vol_elems) fail(YK_ERR_INVALID_BUFFER, "volume buffer smaller than the grid");
+    hipStream_t s = ctx->stream;
+    float* d_vol = volume;
+    if (vloc == YK_HOST) {
+        d_volume_ws.resize(vol_elems);
+        d_vol = d_volume_ws.get();
+        if (received == 0 && !clear_output)
+            YK_HIP(hipMemcpyAsync(d_vol, volume, vol_elems * 4, hipMemcpyHostToDevice, s));
+    }
+    const int chunk = std::min(batch, count);
+    const size_t eb = storage_bytes(desc.storage);
+    d_filtered_ws.resize(size_t(chunk) * view_elems * eb);
+    if (desc.storage != YK_F32) d_scales_ws.resize(size_t(chunk) * nv);
+    if (ploc == YK_HOST) d_proj_ws.resize(size_t(chunk) * view_elems);
+    for (int base = 0; base < count; base += chunk) {
+        const int k = std::min(chunk, count - base);
+        const float* d_proj = projection + size_t(base) * view_elems;
+        if (ploc == YK_HOST) {
+            YK_HIP(hipMemcpyAsync(d_proj_ws.get(), d_proj, size_t(k) * view_elems * 4,
+                hipMemcpyHostToDevice, s));
+            d_proj = d_proj_ws.get();
+        }
+        filter(d_proj, offset + base, k, d_filtered_ws.get(), desc.storage,
+            desc.storage == YK_F32 ? nullptr : d_scales_ws.get());
+        const int acc = (clear_output && base == 0) ? 0 : 1;
+        backproject(d_filtered_ws.get(), desc.storage,
+            desc.storage == YK_F32 ? nullptr : d_scales_ws.get(), offset + base, k, d_vol, acc);
+        if (ploc == YK_HOST) YK_HIP(hipStreamSynchronize(s));  // staging buffer reuse
+    }
+    received += count;
+    if (vloc == YK_HOST && received == n_views)
+        YK_HIP(hipMemcpyAsync(volume, d_vol, vol_elems * 4, hipMemcpyDeviceToHost, s));
+    if (vloc == YK_HOST) YK_HIP(hipStreamSynchronize(s));
+}

@@ -1,0 +1,76 @@
+// FDK plan and projection-operator plan (library-owned handles of the C ABI).
+#pragma once
+
+#include <vector>
+
+#include "yk_internal.hpp"
+
+struct yk_fdk {
+    yk_fdk(yk_ctx* ctx, const yk_fdk_desc& desc, const yk_view* views, int n_views, int nu,
+        int nv, const yk_volume_grid& grid);
+
+    void filter(const float* d_proj, int first, int count, void* d_out, int storage, float* d_scales);
+    void backproject(const void* d_filtered, int storage, const float* d_scales, int first, int count,
+        float* d_volume, int accumulate);
+    void execute(const float* projection, int ploc, size_t pcap, int offset, int count, float* volume,
+        int vloc, size_t vcap, int clear_output);
+
+    yk_ctx* ctx;
+    yk_fdk_desc desc;
+    int nu, nv, n_views;
+    yk_volume_grid grid;
+    yk::VolumeGeom vol{};
+    double scan_range = 0.0;
+    bool parker = false;
+    bool general = false;
+    std::vector<yk_view> views;
+    std::vector<yk::DerivedD> derived;
+    std::vector<yk::CoeffD> coeffs;
+    std::vector<float> weights;
+    int n_pad = 0;
+
+    yk::DeviceArray<float> d_weights;
+    yk::DeviceArray<float2> d_twiddles;
+    yk::DeviceArray<yk::FilterView> d_fviews;
+    yk::DeviceArray<yk::BpCoeff> d_coeffs;
+    yk::DeviceArray<double> d_c64;
+    yk::DeviceArray<float> d_parker;
+    yk::DeviceArray<short4> d_tiles;
+    yk::DeviceArray<int> d_maxes;      // [0..3] footprint maxima, [4] global row-scale bits
+    int nbx = 0, nby = 0, nbz = 0, bz = 0;
+    int lds_quads = 0;
+
+    int batch = 0;
+    yk::DeviceArray<float> d_proj_ws;
+    yk::DeviceArray<unsigned char> d_filtered_ws;
+    yk::DeviceArray<float> d_scales_ws;
+    yk::DeviceArray<float> d_volume_ws;
+    int received = 0;
+
+private:
+    void build_tiles();
+};
+
+struct yk_projector {
+    yk_projector(yk_ctx* ctx, const yk_view* views, int n_views, int nu, int nv,
+        const yk_volume_grid& grid, float scan_range);
+
+    void project(int kind, const float* d_volume, float* d_proj, int first, int count, int accumulate);
+    void backproject(int kind, const float* d_proj, float* d_volume, int first, int count, int accumulate);
+    double dot(const float* a, const float* b, size_t n);
+
+    yk_ctx* ctx;
+    int nu, nv, n_views;
+    yk_volume_grid grid;
+    yk::VolumeGeom vol{};
+    double scan_range;
+    std::vector<yk_view> views;
+    bool any_x_axis = false;
+    yk::DeviceArray<yk::ProjViewDev> d_jviews;
+    yk::DeviceArray<yk::SiddonViewDev> d_sviews;
+    yk::DeviceArray<double> d_c64;
+    yk::DeviceArray<float> d_vol_t;
+    yk::DeviceArray<double> d_partials;
+    yk_fdk* fdk_op = nullptr;          // FDK operator-form backprojector, built on demand
+    ~yk_projector();
+};

@@ -1,0 +1,440 @@
+// Flat-panel forward projectors and their exact transposes for gfx950.
+//
+// Joseph: per ray, sum over every slice of the per-view main axis of the
+// bilinear (minor-axis) sample of the volume at the ray's crossing, times
+// sqrt(v0^2 + (a1 v1)^2 + (a2 v2)^2) (fp_joseph_kernel,
+// src/FlatFpBp/FP/kernels/YkFPKernels.cu:10-202; axis conventions
+// src/common/cuda/operators/YkOperatorKernelTypes.cuh:133-275).  The volume is
+// read with software bilinear in place of a 3-D texture; for X-main-axis
+// views a [z][x][y] copy keeps the minor-axis gathers contiguous.
+//
+// Siddon: exact voxel walk from the source to the pixel centre clipped to
+// t in [0, 1] and the volume box (siddon_fp_kernel,
+// src/FlatFpBp/FP/kernels/YkFPSiddonKernel.cu:19-124); voxel boundaries are
+// recomputed from integer indices instead of accumulated.
+//
+// Transposes are deterministic gathers (no atomics): for each voxel and view
+// the candidate rays are the detector pixels inside the projection of the
+// voxel's support, and each candidate ray is re-evaluated with exactly the
+// arithmetic of the forward kernel (shared functions below, FP contraction
+// off), so <A x, y> = <x, A^T y> up to summation order.
+#include <hip/hip_runtime.h>
+
+#include "yk_internal.hpp"
+
+namespace yk {
+namespace {
+
+struct AxisInfo {
+    int p0, p1, p2;     // component index of c0, c1, c2
+    int n0, n1, n2;
+    float v0, v1, v2;
+};
+
+__device__ __forceinline__ AxisInfo axis_info(int axis, const VolumeGeom& g)
+{
+    AxisInfo a;
+    if (axis == 0) { a.p0 = 0; a.p1 = 1; a.p2 = 2; a.n0 = g.nx; a.n1 = g.ny; a.n2 = g.nz; a.v0 = g.vx; a.v1 = g.vy; a.v2 = g.vz; }
+    else if (axis == 1) { a.p0 = 1; a.p1 = 0; a.p2 = 2; a.n0 = g.ny; a.n1 = g.nx; a.n2 = g.nz; a.v0 = g.vy; a.v1 = g.vx; a.v2 = g.vz; }
+    else { a.p0 = 2; a.p1 = 0; a.p2 = 1; a.n0 = g.nz; a.n1 = g.nx; a.n2 = g.ny; a.v0 = g.vz; a.v1 = g.vx; a.v2 = g.vy; }
+    return a;
+}
+
+__device__ __forceinline__ float comp(const float* v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+
+struct JRay {
+    float a1, b1, a2, b2, L;
+};
+
+// Per-ray Joseph line in volume-centred voxel units.  Shared verbatim by the
+// forward kernel and the transpose, with contraction disabled so both
+// evaluate bit-identical parameters.
+__device__ __attribute__((noinline)) bool joseph_ray(const ProjViewDev& pv, const AxisInfo& ax,
+    int u, int v, JRay& r)
+{
+#pragma clang fp contract(off)
+    float det[3];
+    for (int k = 0; k < 3; ++k) det[k] = (pv.detS[k] + float(u) * pv.detU[k]) + float(v) * pv.detV[k];
+    const float s0 = comp(pv.src, ax.p0), s1 = comp(pv.src, ax.p1), s2 = comp(pv.src, ax.p2);
+    const float d0 = comp(det, ax.p0), d1 = comp(det, ax.p1), d2 = comp(det, ax.p2);
+    const float den = s0 - d0;
+    if (den == 0.f) return false;
+    r.a1 = (s1 - d1) / den;
+    r.a2 = (s2 - d2) / den;
+    r.b1 = s1 - r.a1 * s0;
+    r.b2 = s2 - r.a2 * s0;
+    const float t1 = r.a1 * ax.v1, t2 = r.a2 * ax.v2;
+    r.L = sqrtf(ax.v0 * ax.v0 + t1 * t1 + t2 * t2);
+    return true;
+}
+
+// minor-axis sample position of slice s (index space, texel k at k)
+__device__ __forceinline__ void joseph_sample(const JRay& r, const AxisInfo& ax, int s, float& f1, float& f2)
+{
+#pragma clang fp contract(off)
+    const float c0 = float(s) - 0.5f * float(ax.n0 - 1);
+    f1 = (r.a1 * c0 + r.b1) + 0.5f * float(ax.n1 - 1);
+    f2 = (r.a2 * c0 + r.b2) + 0.5f * float(ax.n2 - 1);
+}
+
+// slices whose sample can touch the volume: -1 < f < n on both minor axes
+__device__ __forceinline__ void joseph_slice_range(const JRay& r, const AxisInfo& ax, int& lo, int& hi)
+{
+    float slo = 0.f, shi = float(ax.n0 - 1);
+    const float h0 = 0.5f * float(ax.n0 - 1);
+    const float a[2] = { r.a1, r.a2 };
+    const float b[2] = { r.b1 + 0.5f * float(ax.n1 - 1), r.b2 + 0.5f * float(ax.n2 - 1) };
+    const float n[2] = { float(ax.n1), float(ax.n2) };
+    for (int k = 0; k < 2; ++k) {
+        if (fabsf(a[k]) > 1e-12f) {
+            float t0 = (-1.f - b[k]) / a[k] + h0;
+            float t1 = (n[k] - b[k]) / a[k] + h0;
+            if (t0 > t1) { const float t = t0; t0 = t1; t1 = t; }
+            slo = fmaxf(slo, t0);
+            shi = fminf(shi, t1);
+        } else if (b[k] <= -1.f || b[k] >= n[k]) {
+            lo = 1; hi = 0;
+            return;
+        }
+    }
+    // one slice of margin on each side; out-of-range samples weigh zero anyway
+    lo = max(0, int(floorf(slo)) - 1);
+    hi = min(ax.n0 - 1, int(ceilf(shi)) + 1);
+}
+
+__global__ void __launch_bounds__(256)
+joseph_fp_kernel(const float* __restrict__ vol, const float* __restrict__ vol_t,
+    float* __restrict__ proj, const ProjViewDev* __restrict__ views, VolumeGeom g,
+    int nu, int nv, int accumulate)
+{
+    const int u = blockIdx.x * 64 + threadIdx.x;
+    const int v = blockIdx.y * 4 + threadIdx.y;
+    const int i = blockIdx.z;
+    if (u >= nu || v >= nv) return;
+    const ProjViewDev pv = views[i];
+    const AxisInfo ax = axis_info(pv.axis, g);
+    // memory strides of (s, i1, i2)
+    const float* base;
+    size_t st0, st2;
+    if (pv.axis == 0) { base = vol_t; st0 = (size_t)g.ny; st2 = (size_t)g.nx * g.ny; }
+    else if (pv.axis == 1) { base = vol; st0 = (size_t)g.nx; st2 = (size_t)g.nx * g.ny; }
+    else { base = vol; st0 = (size_t)g.nx * g.ny; st2 = (size_t)g.nx; }
+    float acc = 0.f;
+    JRay r;
+    float out = 0.f;
+    if (joseph_ray(pv, ax, u, v, r)) {
+        int lo, hi;
+        joseph_slice_range(r, ax, lo, hi);
+        for (int s = lo; s <= hi; ++s) {
+            float f1, f2;
+            joseph_sample(r, ax, s, f1, f2);
+            const float g1 = floorf(f1), g2 = floorf(f2);
+            if (g1 < -1.f || g2 < -1.f || g1 > float(ax.n1 - 1) || g2 > float(ax.n2 - 1)) continue;
+            const int i1 = int(g1), i2 = int(g2);
+            const float w1 = f1 - g1, w2 = f2 - g2;
+            const float* sl = base + (size_t)s * st0;
+            const bool ok1a = i1 >= 0, ok1b = i1 + 1 < ax.n1;
+            const bool ok2a = i2 >= 0, ok2b = i2 + 1 < ax.n2;
+            const float t00 = (ok1a && ok2a) ? sl[(size_t)i2 * st2 + i1] : 0.f;
+            const float t01 = (ok1b && ok2a) ? sl[(size_t)i2 * st2 + i1 + 1] : 0.f;
+            const float t10 = (ok1a && ok2b) ? sl[(size_t)(i2 + 1) * st2 + i1] : 0.f;
+            const float t11 = (ok1b && ok2b) ? sl[(size_t)(i2 + 1) * st2 + i1 + 1] : 0.f;
+            const float r0 = (1.f - w1) * t00 + w1 * t01;
+            const float r1 = (1.f - w1) * t10 + w1 * t11;
+            acc += (1.f - w2) * r0 + w2 * r1;
+        }
+        out = acc * r.L;
+    }
+    float* dst = proj + ((size_t)i * nv + v) * nu + u;
+    *dst = accumulate ? *dst + out : out;
+}
+
+// [z][y][x] -> [z][x][y]
+__global__ void transpose_xy_kernel(const float* __restrict__ in, float* __restrict__ out,
+    int nx, int ny)
+{
+    __shared__ float tile[32][33];
+    const int z = blockIdx.z;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const size_t plane = (size_t)nx * ny;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int y = y0 + k, x = x0 + threadIdx.x;
+        if (x < nx && y < ny) tile[k][threadIdx.x] = in[z * plane + (size_t)y * nx + x];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int x = x0 + k, y = y0 + threadIdx.x;
+        if (x < nx && y < ny) out[z * plane + (size_t)x * ny + y] = tile[threadIdx.x][k];
+    }
+}
+
+// projective map of a world point to detector pixel coordinates (FP64)
+__device__ __forceinline__ bool project64(const double* c, double X, double Y, double Z, double& u, double& v)
+{
+    const double den = c[8] * X + c[9] * Y + c[10] * Z + c[11];
+    if (!(den > 1e-9)) return false;
+    u = (c[0] * X + c[1] * Y + c[2] * Z + c[3]) / den;
+    v = (c[4] * X + c[5] * Y + c[6] * Z + c[7]) / den;
+    return true;
+}
+
+// Joseph transpose: one thread per voxel, gather over views and candidate rays
+__global__ void __launch_bounds__(256)
+joseph_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume,
+    const ProjViewDev* __restrict__ views, const double* __restrict__ c64, VolumeGeom g,
+    int nu, int nv, int count, int accumulate)
+{
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny) return;
+    const int idx3[3] = { x, y, z };
+    const int dims[3] = { g.nx, g.ny, g.nz };
+    const double vox[3] = { g.vx, g.vy, g.vz };
+    const double org[3] = { g.ox, g.oy, g.oz };
+    const float margin = 0.05f;
+    float acc = 0.f;
+    for (int i = 0; i < count; ++i) {
+        const ProjViewDev pv = views[i];
+        const AxisInfo ax = axis_info(pv.axis, g);
+        const int s = idx3[ax.p0], i1 = idx3[ax.p1], i2 = idx3[ax.p2];
+        const double* c = c64 + (size_t)i * 12;
+        // support of the voxel's bilinear weight on slice s: minor index in (i-1, i+1)
+        double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+        bool ok = true;
+        for (int k = 0; k < 4; ++k) {
+            double p[3];
+            p[ax.p0] = org[ax.p0] + double(s) * vox[ax.p0];
+            p[ax.p1] = org[ax.p1] + double(i1 + ((k & 1) ? 1 : -1)) * vox[ax.p1];
+            p[ax.p2] = org[ax.p2] + double(i2 + ((k & 2) ? 1 : -1)) * vox[ax.p2];
+            double uu, vv;
+            if (!project64(c, p[0], p[1], p[2], uu, vv)) { ok = false; break; }
+            umin = fmin(umin, uu); umax = fmax(umax, uu);
+            vmin = fmin(vmin, vv); vmax = fmax(vmax, vv);
+        }
+        if (!ok) continue;
+        const int ulo = max(0, int(ceil(umin - margin)));
+        const int uhi = min(nu - 1, int(floor(umax + margin)));
+        const int vlo = max(0, int(ceil(vmin - margin)));
+        const int vhi = min(nv - 1, int(floor(vmax + margin)));
+        const float* pi = proj + (size_t)i * nu * nv;
+        for (int vv = vlo; vv <= vhi; ++vv)
+            for (int uu = ulo; uu <= uhi; ++uu) {
+                JRay r;
+                if (!joseph_ray(pv, ax, uu, vv, r)) continue;
+                float f1, f2;
+                joseph_sample(r, ax, s, f1, f2);
+                const float g1 = floorf(f1), g2 = floorf(f2);
+                if (g1 < -1.f || g2 < -1.f || g1 > float(ax.n1 - 1) || g2 > float(ax.n2 - 1)) continue;
+                const int j1 = int(g1), j2 = int(g2);
+                const float w1 = f1 - g1, w2 = f2 - g2;
+                float a;
+                if (i1 == j1) a = 1.f - w1;
+                else if (i1 == j1 + 1) a = w1;
+                else continue;
+                float b;
+                if (i2 == j2) b = 1.f - w2;
+                else if (i2 == j2 + 1) b = w2;
+                else continue;
+                acc += pi[(size_t)vv * nu + uu] * r.L * (a * b);
+            }
+    }
+    float* dst = volume + ((size_t)z * g.ny + y) * g.nx + x;
+    *dst = accumulate ? *dst + acc : acc;
+    (void)dims;
+}
+
+// ---------------------------------------------------------------------------
+// Siddon
+// ---------------------------------------------------------------------------
+struct Box {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ Box volume_box(const VolumeGeom& g)
+{
+    Box b;
+    b.lo[0] = g.ox - 0.5f * g.vx; b.hi[0] = g.ox + (float(g.nx) - 0.5f) * g.vx;
+    b.lo[1] = g.oy - 0.5f * g.vy; b.hi[1] = g.oy + (float(g.ny) - 0.5f) * g.vy;
+    b.lo[2] = g.oz - 0.5f * g.vz; b.hi[2] = g.oz + (float(g.nz) - 0.5f) * g.vz;
+    return b;
+}
+
+// parametric segment [t0, t1] of src + t ray inside the box, clipped to [0, 1]
+__device__ __forceinline__ bool clip_ray(const float* src, const float* ray, const float* lo,
+    const float* hi, float& t0, float& t1)
+{
+    t0 = 0.f;
+    t1 = 1.f;
+    for (int k = 0; k < 3; ++k) {
+        if (fabsf(ray[k]) > 1e-8f) {
+            float ta = (lo[k] - src[k]) / ray[k], tb = (hi[k] - src[k]) / ray[k];
+            if (ta > tb) { const float t = ta; ta = tb; tb = t; }
+            t0 = fmaxf(t0, ta);
+            t1 = fminf(t1, tb);
+        } else if (src[k] < lo[k] || src[k] > hi[k]) {
+            return false;
+        }
+    }
+    return t0 < t1;
+}
+
+__device__ __forceinline__ void siddon_ray_geom(const SiddonViewDev& sv, int u, int v, float* src, float* ray)
+{
+    for (int k = 0; k < 3; ++k) {
+        src[k] = sv.src[k];
+        ray[k] = (sv.detS[k] + float(u) * sv.detU[k] + float(v) * sv.detV[k]) - sv.src[k];
+    }
+}
+
+__global__ void __launch_bounds__(256)
+siddon_fp_kernel(const float* __restrict__ vol, float* __restrict__ proj,
+    const SiddonViewDev* __restrict__ views, VolumeGeom g, int nu, int nv, int accumulate)
+{
+    const int u = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int v = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int i = blockIdx.z;
+    if (u >= nu || v >= nv) return;
+    const SiddonViewDev sv = views[i];
+    float src[3], ray[3];
+    siddon_ray_geom(sv, u, v, src, ray);
+    const Box b = volume_box(g);
+    const int dims[3] = { g.nx, g.ny, g.nz };
+    const float vox[3] = { g.vx, g.vy, g.vz };
+    float t0, t1, acc = 0.f;
+    if (clip_ray(src, ray, b.lo, b.hi, t0, t1)) {
+        const float len = sqrtf(ray[0] * ray[0] + ray[1] * ray[1] + ray[2] * ray[2]);
+        int idx[3], step[3];
+        float tn[3], rcp[3];
+        bool valid[3];
+        for (int k = 0; k < 3; ++k) {
+            const float e = src[k] + t0 * ray[k];
+            idx[k] = min(dims[k] - 1, max(0, int(floorf((e - b.lo[k]) / vox[k]))));
+            step[k] = ray[k] >= 0.f ? 1 : -1;
+            valid[k] = fabsf(ray[k]) > 1e-8f;
+            rcp[k] = valid[k] ? 1.f / ray[k] : 0.f;
+            const float plane = b.lo[k] + float(idx[k] + (step[k] > 0 ? 1 : 0)) * vox[k];
+            tn[k] = valid[k] ? (plane - src[k]) * rcp[k] : 3.0e38f;
+        }
+        float t = t0;
+        const int max_steps = g.nx + g.ny + g.nz + 3;
+        for (int n = 0; n < max_steps && t < t1; ++n) {
+            const float tnext = fminf(fminf(tn[0], tn[1]), fminf(tn[2], t1));
+            acc += vol[((size_t)idx[2] * g.ny + idx[1]) * g.nx + idx[0]] * (tnext - t);
+            t = tnext;
+            int k;
+            if (tn[0] <= tn[1] && tn[0] <= tn[2]) k = 0;
+            else if (tn[1] <= tn[2]) k = 1;
+            else k = 2;
+            idx[k] += step[k];
+            if (idx[k] < 0 || idx[k] >= dims[k]) break;
+            const float plane = b.lo[k] + float(idx[k] + (step[k] > 0 ? 1 : 0)) * vox[k];
+            tn[k] = (plane - src[k]) * rcp[k];
+        }
+        acc *= len;
+    }
+    float* dst = proj + ((size_t)i * nv + v) * nu + u;
+    *dst = accumulate ? *dst + acc : acc;
+}
+
+// Siddon transpose: one thread per voxel; for each view the candidate rays
+// are the pixels inside the projection of the voxel cube, and each candidate
+// contributes y * |segment inside the cube, clipped like the forward walk|.
+__global__ void __launch_bounds__(256)
+siddon_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume,
+    const SiddonViewDev* __restrict__ views, const double* __restrict__ c64, VolumeGeom g,
+    int nu, int nv, int count, int accumulate)
+{
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny) return;
+    const Box b = volume_box(g);
+    const float vox[3] = { g.vx, g.vy, g.vz };
+    const int idx3[3] = { x, y, z };
+    float clo[3], chi[3];
+    for (int k = 0; k < 3; ++k) {
+        clo[k] = b.lo[k] + float(idx3[k]) * vox[k];
+        chi[k] = b.lo[k] + float(idx3[k] + 1) * vox[k];
+    }
+    const float margin = 0.05f;
+    float acc = 0.f;
+    for (int i = 0; i < count; ++i) {
+        const double* c = c64 + (size_t)i * 12;
+        double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+        bool ok = true;
+        for (int k = 0; k < 8; ++k) {
+            const double X = (k & 1) ? chi[0] : clo[0];
+            const double Y = (k & 2) ? chi[1] : clo[1];
+            const double Z = (k & 4) ? chi[2] : clo[2];
+            double uu, vv;
+            if (!project64(c, X, Y, Z, uu, vv)) { ok = false; break; }
+            umin = fmin(umin, uu); umax = fmax(umax, uu);
+            vmin = fmin(vmin, vv); vmax = fmax(vmax, vv);
+        }
+        if (!ok) continue;
+        const int ulo = max(0, int(ceil(umin - margin)));
+        const int uhi = min(nu - 1, int(floor(umax + margin)));
+        const int vlo = max(0, int(ceil(vmin - margin)));
+        const int vhi = min(nv - 1, int(floor(vmax + margin)));
+        if (ulo > uhi || vlo > vhi) continue;
+        const SiddonViewDev sv = views[i];
+        const float* pi = proj + (size_t)i * nu * nv;
+        for (int vv = vlo; vv <= vhi; ++vv)
+            for (int uu = ulo; uu <= uhi; ++uu) {
+                float src[3], ray[3], t0, t1, c0, c1;
+                siddon_ray_geom(sv, uu, vv, src, ray);
+                if (!clip_ray(src, ray, b.lo, b.hi, t0, t1)) continue;
+                if (!clip_ray(src, ray, clo, chi, c0, c1)) continue;
+                const float a = fmaxf(t0, c0), e = fminf(t1, c1);
+                if (!(e > a)) continue;
+                const float len = sqrtf(ray[0] * ray[0] + ray[1] * ray[1] + ray[2] * ray[2]);
+                acc += pi[(size_t)vv * nu + uu] * ((e - a) * len);
+            }
+    }
+    float* dst = volume + ((size_t)z * g.ny + y) * g.nx + x;
+    *dst = accumulate ? *dst + acc : acc;
+}
+
+}  // namespace
+
+void launch_joseph_fp(const JosephLaunch& p, hipStream_t s)
+{
+    dim3 block(64, 4), grid((p.nu + 63) / 64, (p.nv + 3) / 4, p.count);
+    hipLaunchKernelGGL(joseph_fp_kernel, grid, block, 0, s, p.volume, p.volume_t, p.proj, p.views,
+        p.vol, p.nu, p.nv, p.accumulate);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_transpose_xy(const float* in, float* out, int nx, int ny, int nz, hipStream_t s)
+{
+    dim3 block(32, 8), grid((nx + 31) / 32, (ny + 31) / 32, nz);
+    hipLaunchKernelGGL(transpose_xy_kernel, grid, block, 0, s, in, out, nx, ny);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_joseph_adjoint(const JosephAdjointLaunch& p, hipStream_t s)
+{
+    dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, p.vol.nz);
+    hipLaunchKernelGGL(joseph_adjoint_kernel, grid, block, 0, s, p.proj, p.volume, p.views, p.coeff64,
+        p.vol, p.nu, p.nv, p.count, p.accumulate);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_siddon_fp(const SiddonLaunch& p, hipStream_t s)
+{
+    dim3 block(256), grid((p.nu + 15) / 16, (p.nv + 15) / 16, p.count);
+    hipLaunchKernelGGL(siddon_fp_kernel, grid, block, 0, s, p.volume, p.proj, p.views, p.vol, p.nu,
+        p.nv, p.accumulate);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_siddon_adjoint(const SiddonAdjointLaunch& p, hipStream_t s)
+{
+    dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, p.vol.nz);
+    hipLaunchKernelGGL(siddon_adjoint_kernel, grid, block, 0, s, p.proj, p.volume, p.views, p.coeff64,
+        p.vol, p.nu, p.nv, p.count, p.accumulate);
+    YK_HIP(hipGetLastError());
+}
+
+}  // namespace yk

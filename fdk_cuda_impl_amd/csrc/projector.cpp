@@ -1,0 +1,146 @@
+// Projection-operator plan: per-view device records for the Joseph and
+// Siddon forward models and their transposes, and the iterative callers
+// (CGLS, SART) built on them.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fdk.hpp"
+
+using namespace yk;
+
+yk_projector::yk_projector(yk_ctx* c, const yk_view* v, int n, int nu_, int nv_,
+    const yk_volume_grid& g, float range)
+    : ctx(c), nu(nu_), nv(nv_), n_views(n), grid(g),
+      scan_range(range > 0.f ? double(range) : 6.28318530717958647692)
+{
+    if (!v || n <= 0) fail(YK_ERR_INVALID_GEOMETRY, "projector needs at least one view");
+    if (nu <= 0 || nv <= 0) fail(YK_ERR_INVALID_GEOMETRY, "invalid detector size");
+    validate_volume(grid);
+    views.assign(v, v + n);
+    vol = volume_geom(grid);
+    ctx->activate();
+    std::vector<ProjViewDev> jv(n);
+    std::vector<SiddonViewDev> sv(n);
+    std::vector<double> c64(size_t(n) * 12);
+    const double vox[3] = { grid.voxel_x_mm, grid.voxel_y_mm, grid.voxel_z_mm };
+    const double cen[3] = { grid.center_mm[0], grid.center_mm[1], grid.center_mm[2] };
+    for (int i = 0; i < n; ++i) {
+        const yk_view& w = views[i];
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(w.src[k]) || !std::isfinite(w.detS[k]) || !std::isfinite(w.detU[k]) ||
+                !std::isfinite(w.detV[k]))
+                fail(YK_ERR_INVALID_GEOMETRY, "non-finite view vector");
+        // normalizeToVoxel (src/common/cuda/operators/YkOperatorKernelTypes.cuh:148-168)
+        ProjViewDev& p = jv[i];
+        std::memset(&p, 0, sizeof(p));
+        for (int k = 0; k < 3; ++k) {
+            p.src[k] = float((double(w.src[k]) - cen[k]) / vox[k]);
+            p.detS[k] = float((double(w.detS[k]) - cen[k]) / vox[k]);
+            p.detU[k] = float(double(w.detU[k]) / vox[k]);
+            p.detV[k] = float(double(w.detV[k]) / vox[k]);
+        }
+        p.axis = main_axis(w, nu, nv);
+        if (p.axis == 0) any_x_axis = true;
+        SiddonViewDev& s = sv[i];
+        std::memset(&s, 0, sizeof(s));
+        std::memcpy(s.src, w.src, sizeof(s.src));
+        std::memcpy(s.detS, w.detS, sizeof(s.detS));
+        std::memcpy(s.detU, w.detU, sizeof(s.detU));
+        std::memcpy(s.detV, w.detV, sizeof(s.detV));
+        projection_map(w, &c64[size_t(i) * 12]);
+    }
+    hipStream_t s = ctx->stream;
+    d_jviews.upload(jv.data(), jv.size(), s);
+    d_sviews.upload(sv.data(), sv.size(), s);
+    d_c64.upload(c64.data(), c64.size(), s);
+    d_partials.resize(1024);
+    YK_HIP(hipStreamSynchronize(s));
+}
+
+yk_projector::~yk_projector()
+{
+    delete fdk_op;
+}
+
+void yk_projector::project(int kind, const float* d_volume, float* d_proj, int first, int count,
+    int accumulate)
+{
+    if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
+    if (!d_volume || !d_proj) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
+    hipStream_t s = ctx->stream;
+    if (kind == YK_FP_JOSEPH) {
+        bool need_t = false;
+        std::vector<ProjViewDev> dummy;
+        for (int i = first; i < first + count; ++i)
+            if (main_axis(views[i], nu, nv) == 0) need_t = true;
+        if (need_t) {
+            d_vol_t.resize(size_t(grid.nx) * grid.ny * grid.nz);
+            launch_transpose_xy(d_volume, d_vol_t.get(), grid.nx, grid.ny, grid.nz, s);
+        }
+        JosephLaunch p{};
+        p.volume = d_volume;
+        p.volume_t = need_t ? d_vol_t.get() : nullptr;
+        p.proj = d_proj;
+        p.views = d_jviews.get() + first;
+        p.vol = vol;
+        p.nu = nu; p.nv = nv; p.count = count;
+        p.accumulate = accumulate;
+        launch_joseph_fp(p, s);
+    } else if (kind == YK_FP_SIDDON) {
+        SiddonLaunch p{};
+        p.volume = d_volume;
+        p.proj = d_proj;
+        p.views = d_sviews.get() + first;
+        p.vol = vol;
+        p.nu = nu; p.nv = nv; p.count = count;
+        p.accumulate = accumulate;
+        launch_siddon_fp(p, s);
+    } else {
+        fail(YK_ERR_UNSUPPORTED, "unknown forward projector");
+    }
+}
+
+void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, int first, int count,
+    int accumulate)
+{
+    if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
+    if (!d_volume || !d_proj) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
+    hipStream_t s = ctx->stream;
+    if (kind == YK_BP_JOSEPH_ADJOINT) {
+        JosephAdjointLaunch p{};
+        p.proj = d_proj;
+        p.volume = d_volume;
+        p.views = d_jviews.get() + first;
+        p.coeff64 = d_c64.get() + size_t(first) * 12;
+        p.vol = vol;
+        p.nu = nu; p.nv = nv; p.count = count;
+        p.accumulate = accumulate;
+        launch_joseph_adjoint(p, s);
+    } else if (kind == YK_BP_SIDDON_ADJOINT) {
+        SiddonAdjointLaunch p{};
+        p.proj = d_proj;
+        p.volume = d_volume;
+        p.views = d_sviews.get() + first;
+        p.coeff64 = d_c64.get() + size_t(first) * 12;
+        p.vol = vol;
+        p.nu = nu; p.nv = nv; p.count = count;
+        p.accumulate = accumulate;
+        launch_siddon_adjoint(p, s);
+    } else {
+        fail(YK_ERR_UNSUPPORTED, "unknown back projector");
+    }
+}
+
+double yk_projector::dot(const float* a, const float* b, size_t n)
+{
+    hipStream_t s = ctx->stream;
+    const int parts = int(std::min<size_t>(1024, std::max<size_t>(1, (n + 255) / 256)));
+    launch_dot(a, b, n, d_partials.get(), parts, s);
+    std::vector<double> h(parts);
+    YK_HIP(hipMemcpyAsync(h.data(), d_partials.get(), parts * sizeof(double), hipMemcpyDeviceToHost, s));
+    YK_HIP(hipStreamSynchronize(s));
+    double acc = 0.0;
+    for (double v : h) acc += v;
+    return acc;
+}

@@ -1,0 +1,307 @@
+// Internal host-side declarations shared by the C ABI, the plans and the
+// kernel launchers of libyk_hip.so.  Nothing here is part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/yk_hip.h"
+
+namespace yk {
+
+// ---------------------------------------------------------------------------
+// errors: every failure is a yk::Error carrying a yk_status; the C ABI layer
+// converts it to a return code and a thread-local message (no aborts).
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    yk_status status;
+    Error(yk_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+};
+
+[[noreturn]] void fail(yk_status status, const std::string& message);
+void check_hip(hipError_t err, const char* what);
+
+#define YK_HIP(call) ::yk::check_hip((call), #call)
+
+// ---------------------------------------------------------------------------
+// device memory RAII (hipMalloc'd, freed on destruction)
+// ---------------------------------------------------------------------------
+template <typename T>
+class DeviceArray {
+public:
+    DeviceArray() = default;
+    explicit DeviceArray(size_t n) { resize(n); }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    DeviceArray(DeviceArray&& o) noexcept : ptr_(o.ptr_), n_(o.n_) { o.ptr_ = nullptr; o.n_ = 0; }
+    DeviceArray& operator=(DeviceArray&& o) noexcept
+    {
+        if (this != &o) { reset(); ptr_ = o.ptr_; n_ = o.n_; o.ptr_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    ~DeviceArray() { reset(); }
+    void resize(size_t n)
+    {
+        if (n == n_) return;
+        reset();
+        if (n) {
+            void* p = nullptr;
+            YK_HIP(hipMalloc(&p, n * sizeof(T)));
+            ptr_ = static_cast<T*>(p);
+            n_ = n;
+        }
+    }
+    void reset()
+    {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return ptr_; }
+    size_t size() const { return n_; }
+    void upload(const T* host, size_t n, hipStream_t s)
+    {
+        resize(n);
+        if (n) YK_HIP(hipMemcpyAsync(ptr_, host, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+
+private:
+    T* ptr_ = nullptr;
+    size_t n_ = 0;
+};
+
+// pinned host staging buffer (hipHostMalloc'd)
+class PinnedBuffer {
+public:
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer() { reset(); }
+    void ensure(size_t bytes);
+    void reset();
+    void* get() const { return ptr_; }
+
+private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+}  // namespace yk
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct yk_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    bool synchronous = true;
+    yk::DeviceArray<double> reduce_scratch;
+    yk::PinnedBuffer pinned;
+    void finish();  // synchronizes when synchronous, surfaces async errors
+    void activate() const;
+};
+
+namespace yk {
+
+// ---------------------------------------------------------------------------
+// geometry (FP64 host math; float results)
+// ---------------------------------------------------------------------------
+struct ViewD {
+    double src[3], detS[3], detU[3], detV[3];
+    double angle;
+};
+
+ViewD widen(const yk_view& v);
+
+struct DerivedD {
+    double theta, dtheta, fscale, sod, sdd_radial, sdd_plane, off_u, off_v, du, dv;
+    double radial[3], normal[3];
+    double uu, vv, uv;
+};
+
+struct CoeffD {
+    double cu[4], cv[4], cd[4], cr[4];
+    double wbase;
+};
+
+void build_views(const yk_system& system, std::vector<yk_view>& views);
+void derive_geometry(const std::vector<yk_view>& views, int nu, int nv, double scan_range,
+    std::vector<DerivedD>& out);
+void fdk_coefficients(const std::vector<yk_view>& views, const std::vector<DerivedD>& derived,
+    std::vector<CoeffD>& out);
+void filter_weights(const yk_filter& f, int padded_n, double du, std::vector<float>& out);
+void validate_filter(const yk_filter& f);
+int padded_n(int nu);
+int start_u(int nu, int n, double offset_u_pix);
+int main_axis(const yk_view& v, int nu, int nv);
+// world point -> detector pixel coordinates: u = (c[0..3].[X,1]) / (c[8..11].[X,1]),
+// v = (c[4..7].[X,1]) / (c[8..11].[X,1]); same map as the FDK coefficients
+void projection_map(const yk_view& v, double c[12]);
+
+// ---------------------------------------------------------------------------
+// device-side records (plain float POD passed to kernels)
+// ---------------------------------------------------------------------------
+struct alignas(16) BpCoeff {
+    float cu[4];   // x y z w
+    float cv[4];
+    float cd[4];
+    float cr[4];
+    float wbase;
+    float pad[3];
+};
+
+struct alignas(16) FilterView {
+    float src[4];
+    float detS[4];
+    float detU[4];
+    float detV[4];
+    float sdd_radial;
+    float wgeom;       // SDD_r / SOD / 2
+    int start_u;
+    int pad;
+};
+
+struct VolumeGeom {
+    int nx, ny, nz;
+    float vx, vy, vz;
+    float ox, oy, oz;  // centre of voxel (0,0,0)
+};
+
+VolumeGeom volume_geom(const yk_volume_grid& g);
+void validate_volume(const yk_volume_grid& g);
+
+// ---------------------------------------------------------------------------
+// kernel launchers (implemented in the .hip translation units)
+// ---------------------------------------------------------------------------
+struct FilterLaunch {
+    const float* proj;          // [count][nv][nu]
+    void* out;                  // [count][nv][nu] of `storage`
+    float* row_scales;          // [count][nv] (half / bf16)
+    const FilterView* views;    // device, `count` records
+    const float* parker;        // device [count][nu] Parker weights x range/pi, or null
+    const float* weights;       // device, N/2+1
+    const float2* twiddles;     // device, N
+    int nu, nv, count, n;
+    int storage;
+};
+void launch_fdk_filter(const FilterLaunch& p, hipStream_t s);
+
+struct TileLaunch {
+    const double* coeff64;      // device, per view: cu[4] cv[4] cd[4] (12 doubles)
+    int n_views;
+    VolumeGeom vol;
+    int bx, by, bz;             // voxel block
+    int nbx, nby, nbz;          // block counts
+    int nu, nv;
+    short4* tiles;              // [n_blocks][n_views]: u0, v0, width, height (quads)
+    int* max_quads;             // device scalars: max width, max height, max w*h, bad
+};
+void launch_bp_tiles(const TileLaunch& p, hipStream_t s);
+
+struct BpLaunch {
+    const void* proj;           // [count][nv][nu], storage type
+    const float* row_scales;    // [count][nv] or null
+    const int* gscale_bits;     // device float bits: max row scale (half / bf16)
+    const BpCoeff* coeffs;      // device, `count` records
+    const short4* tiles;        // [n_blocks][n_tile_views] with column offset
+    int tile_view_stride;       // = total views of the plan
+    int tile_view_offset;       // first view in the tile table
+    float* volume;
+    VolumeGeom vol;
+    int nu, nv, count;
+    int storage;
+    int accumulate;
+    int kahan;
+    int general;                // 1: u varies along z (tilted detector)
+    int nbx, nby, nbz;
+    int bz;                     // z slices per workgroup (32, 16 or 8)
+    int lds_quads;              // LDS capacity in quads
+};
+void launch_fdk_bp(const BpLaunch& p, hipStream_t s);
+void launch_max_scale(const float* scales, int n, int* out_bits, hipStream_t s);
+int bp_block_z(int general);
+
+struct ProjViewDev {
+    float src[4];               // voxel-normalized, volume-centred
+    float detS[4];
+    float detU[4];
+    float detV[4];
+    int axis;                   // 0 X, 1 Y, 2 Z
+    int pad[3];
+};
+
+struct SiddonViewDev {
+    float src[4];               // world
+    float detS[4];
+    float detU[4];
+    float detV[4];
+};
+
+struct JosephLaunch {
+    const float* volume;        // [z][y][x]
+    const float* volume_t;      // [z][x][y] copy, for axis-X views (may be null)
+    float* proj;                // [count][nv][nu]
+    const ProjViewDev* views;   // device, `count`
+    VolumeGeom vol;
+    int nu, nv, count;
+    int accumulate;
+};
+void launch_joseph_fp(const JosephLaunch& p, hipStream_t s);
+void launch_transpose_xy(const float* in, float* out, int nx, int ny, int nz, hipStream_t s);
+
+struct JosephAdjointLaunch {
+    const float* proj;          // [count][nv][nu]
+    float* volume;              // [z][y][x]
+    const ProjViewDev* views;   // device, `count`
+    const double* coeff64;      // device, per view cu[4] cv[4] cd[4] (world projective map)
+    VolumeGeom vol;
+    int nu, nv, count;
+    int accumulate;
+};
+void launch_joseph_adjoint(const JosephAdjointLaunch& p, hipStream_t s);
+
+struct SiddonLaunch {
+    const float* volume;
+    float* proj;
+    const SiddonViewDev* views;
+    VolumeGeom vol;
+    int nu, nv, count;
+    int accumulate;
+};
+void launch_siddon_fp(const SiddonLaunch& p, hipStream_t s);
+
+struct SiddonAdjointLaunch {
+    const float* proj;
+    float* volume;
+    const SiddonViewDev* views;
+    const double* coeff64;
+    VolumeGeom vol;
+    int nu, nv, count;
+    int accumulate;
+};
+void launch_siddon_adjoint(const SiddonAdjointLaunch& p, hipStream_t s);
+
+// vector helpers
+void launch_dot(const float* a, const float* b, size_t n, double* partials, int n_partials,
+    hipStream_t s);
+void launch_axpy(float* y, const float* x, float a, size_t n, hipStream_t s);        // y += a x
+void launch_xpby(float* y, const float* x, float b, size_t n, hipStream_t s);        // y = x + b y
+void launch_scale_copy(float* y, const float* x, float a, size_t n, hipStream_t s);  // y = a x
+void launch_fill(float* y, float v, size_t n, hipStream_t s);
+// res = (b - res) / row where row > eps, else 0
+void sart_residual(float* res, const float* b, const float* row, size_t n, hipStream_t s);
+// x += lambda upd / col where col > eps; optional clamp at 0
+void sart_update(float* x, const float* upd, const float* col, float lambda, int nonnegative, size_t n,
+    hipStream_t s);
+void launch_tv_gradient(const float* x, float* grad, double* partials, int n_partials,
+    const VolumeGeom& g, float eps, hipStream_t s);
+void launch_phantom(const float* ellipsoids, int n, const VolumeGeom& g, int clamp,
+    float* out, hipStream_t s);
+
+}  // namespace yk

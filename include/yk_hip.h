@@ -1,0 +1,302 @@
+/*
+ * yk_hip.h -- C ABI of the MI355X (gfx950) flat-panel circular-orbit
+ * reconstruction path: FDK filter + backprojection, Joseph and Siddon forward
+ * projection, their matched transposes, and the CGLS / SART / TV callers.
+ *
+ * This is the drop-in boundary behind the YKCBCT Session API
+ * (include/YKCBCT/interface/YkReconstructionApi.hpp in the reference).
+ * Every entry point is extern "C", takes plain pointers and sizes, returns a
+ * yk_status code and never aborts.  Status values mirror YK::EApiErrorCode
+ * (YkReconstructionApi.hpp:72-81) one for one; YK_ERR_DEVICE stands where the
+ * reference says CudaError.
+ *
+ * Data conventions are the reference's, unchanged
+ * (doc/geometry-and-data.md:39,74,105-110):
+ *   projections [view][v][u] float32, volumes [z][y][x] float32,
+ *   millimetres, attenuation in mm^-1, p = integral of mu dl;
+ *   yk_view.detS is the centre of pixel (0,0), detU/detV one-pixel steps;
+ *   voxel (i,j,k) centre = centre_mm + ((i,j,k) - (N-1)/2) * voxel_mm.
+ *
+ * Threading and lifetime (reference: doc/public-api.md:12-16,137,183-195):
+ *   a yk_ctx owns one HIP stream (or adopts a caller stream) on one device and
+ *   is not thread-safe; separate contexts may be used from separate threads.
+ *   Handles (yk_fdk, yk_projector) are library-owned and must be destroyed
+ *   through this API.  All buffer pointers are caller-owned and borrowed for
+ *   the call.  Device pointers must belong to the context's device.  A call
+ *   is synchronous on success unless yk_ctx_set_synchronous(ctx, 0) was used,
+ *   in which case work is only enqueued on the context stream.
+ */
+#ifndef YK_HIP_H
+#define YK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YK_HIP_ABI_VERSION 1
+
+/* ------------------------------------------------------------------------ */
+/* status, enums                                                             */
+/* ------------------------------------------------------------------------ */
+typedef enum yk_status {
+    YK_OK = 0,
+    YK_ERR_INVALID_CONFIG = 1,
+    YK_ERR_INVALID_GEOMETRY = 2,
+    YK_ERR_UNSUPPORTED = 3,
+    YK_ERR_ALGORITHM_UNDER_TEST = 4,
+    YK_ERR_INVALID_BUFFER = 5,
+    YK_ERR_STREAMING_STATE = 6,
+    YK_ERR_DEVICE = 7
+} yk_status;
+
+/* YK::EMemoryLocation (YkReconstructionTypes.hpp:132-135) */
+typedef enum yk_location { YK_HOST = 0, YK_DEVICE = 1 } yk_location;
+
+/* storage type of filtered projections */
+typedef enum yk_dtype { YK_F32 = 0, YK_F16 = 1, YK_BF16 = 2 } yk_dtype;
+
+/* YK::EFdkFilter (algorithms/analytic/YkAnalyticFilterParams.hpp:5-8) */
+typedef enum yk_filter_type {
+    YK_FILTER_NONE = 0, YK_FILTER_RAMLAK = 1, YK_FILTER_SHEPPLOGAN = 2,
+    YK_FILTER_COSINE = 3, YK_FILTER_HANN = 4, YK_FILTER_HAMMING = 5,
+    YK_FILTER_BLACKMAN = 6, YK_FILTER_BUTTERWORTH = 7, YK_FILTER_KAISER = 8,
+    YK_FILTER_TUKEY = 9
+} yk_filter_type;
+
+/* YK::EParkerMode (algorithms/analytic/YkFdkParams.hpp) */
+typedef enum yk_parker_mode {
+    YK_PARKER_AUTO = 0, YK_PARKER_DISABLED = 1, YK_PARKER_ENABLED = 2
+} yk_parker_mode;
+
+/* forward models, YK::ETask FP_* (src/common/YkOperatorTypes.hpp:5-22) */
+typedef enum yk_forward_kind { YK_FP_JOSEPH = 1, YK_FP_SIDDON = 2 } yk_forward_kind;
+
+/* back models: exact transposes of the forward models, and the FDK
+ * operator-form backprojectors used as B by SART (YkBpFdkKernel.cu) */
+typedef enum yk_back_kind {
+    YK_BP_JOSEPH_ADJOINT = 8, YK_BP_SIDDON_ADJOINT = 4,
+    YK_BP_FDK = 6, YK_BP_FDK_PSEUDO_MATCHED = 7
+} yk_back_kind;
+
+/* ------------------------------------------------------------------------ */
+/* geometry PODs (mirror the reference's public structs field for field)      */
+/* ------------------------------------------------------------------------ */
+/* SConeProjGeomVec (geometry/YkProjectionGeometry.hpp:12-18), 80 bytes */
+typedef struct yk_view {
+    float src[4];
+    float detS[4];
+    float detU[4];
+    float detV[4];
+    float angle[4];  /* angle[0] = acquisition angle, rad */
+} yk_view;
+
+/* SRegularCircularScanSpec (geometry/YkGeometryBuilderTypes.hpp:173-183) */
+typedef struct yk_circular_scan {
+    int total_views;
+    int views_per_turn;
+    float start_angle_rad;
+    int rotation_direction;
+    float sid_mm;
+    float sdd_mm;
+    float z_mm;
+    float source_offset_mm[3];
+} yk_circular_scan;
+
+/* SDetectorPoseSpec (YkGeometryBuilderTypes.hpp:216-223): offset is U, N, V */
+typedef struct yk_detector_pose {
+    float offset_unv_mm[3];
+    float tilt_u_rad;
+    float tilt_v_rad;
+    float tilt_n_rad;
+} yk_detector_pose;
+
+/* SFlatDetectorSpec (YkGeometryBuilderTypes.hpp:232-238) */
+typedef struct yk_flat_detector {
+    int channels;
+    int rows;
+    float channel_size_mm;
+    float row_size_mm;
+    yk_detector_pose pose;
+} yk_flat_detector;
+
+/* SVolumeGridSpec (YkGeometryBuilderTypes.hpp:252-260) */
+typedef struct yk_volume_grid {
+    int nx, ny, nz;
+    float voxel_x_mm, voxel_y_mm, voxel_z_mm;
+    float center_mm[3];
+} yk_volume_grid;
+
+/* Flat + circular subset of SSystemConfig (geometry/YkSystemGeometry.hpp:23-32) */
+typedef struct yk_system {
+    yk_circular_scan scan;
+    yk_flat_detector detector;
+    yk_volume_grid volume;
+} yk_system;
+
+/* SAnalyticFilterParams (algorithms/analytic/YkAnalyticFilterParams.hpp:9-14) */
+typedef struct yk_filter {
+    int type;  /* yk_filter_type */
+    float cutoff;
+    float gain;
+    float butterworth_order;
+    float kaiser_beta;
+    float tukey_alpha;
+} yk_filter;
+
+/* FDK plan options (SFdkAlgoParams + scan range, plus the MI355X storage
+ * choices of this build) */
+typedef struct yk_fdk_desc {
+    uint32_t struct_size;       /* sizeof(yk_fdk_desc) */
+    yk_filter filter;
+    int parker_mode;            /* yk_parker_mode */
+    float scan_range_rad;       /* total angular range; 0 = 2 pi */
+    float start_angle_rad;      /* Parker angle base */
+    int rotation_direction;     /* +1 / -1 */
+    int storage;                /* yk_dtype of filtered projections */
+    int kahan;                  /* compensated summation across views */
+    int batch_views;            /* views filtered per workspace pass; 0 = auto */
+} yk_fdk_desc;
+
+typedef struct yk_ctx yk_ctx;
+typedef struct yk_fdk yk_fdk;
+typedef struct yk_projector yk_projector;
+
+/* ------------------------------------------------------------------------ */
+/* library and context                                                       */
+/* ------------------------------------------------------------------------ */
+/* ABI version of this header */
+int yk_abi_version(void);
+/* message of the last failure on this thread (never NULL) */
+const char* yk_last_error(void);
+
+/* One HIP stream on one device (ResourceContext::initialize,
+ * src/common/YkExecutionContext.hpp:164-170). */
+yk_status yk_ctx_create(int device, yk_ctx** out);
+yk_status yk_ctx_destroy(yk_ctx* ctx);
+/* Adopt a caller-owned HIP stream (hipStream_t, default priority); NULL
+ * means the device's null stream.  yk_ctx_reset_stream returns to the
+ * context's own stream. */
+yk_status yk_ctx_set_stream(yk_ctx* ctx, void* hip_stream);
+yk_status yk_ctx_reset_stream(yk_ctx* ctx);
+void* yk_ctx_stream(yk_ctx* ctx);
+/* 1 (default): every call waits for its GPU work; 0: enqueue only */
+yk_status yk_ctx_set_synchronous(yk_ctx* ctx, int synchronous);
+yk_status yk_ctx_synchronize(yk_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* geometry                                                                  */
+/* ------------------------------------------------------------------------ */
+/* Per-view cone vectors of a regular flat circular scan (buildSystemGeometry
+ * for Flat + Circular, geometry/YkSystemGeometry.hpp:115-131).  Computed in
+ * FP64 and rounded to float.  *count receives total_views. */
+yk_status yk_geometry_build(const yk_system* system, yk_view* out, int capacity,
+    int* count);
+
+/* FDK derived geometry, 20 floats per view in the order theta, dtheta,
+ * fscale, SOD, SDD_radial, SDD_plane, offsetU_pix, offsetV_pix, du, dv,
+ * radial[3], normal[3], UU, VV, UV, 0 (SFDKGeoParamPerView,
+ * src/global/YkFdkKernelTypes.hpp:7-32). */
+yk_status yk_geometry_derive(const yk_view* views, int n_views, int nu, int nv,
+    float scan_range_rad, float* out);
+
+/* FDK affine coefficients, 17 floats per view: Cu[4], Cv[4], Cd[4], Cr[4],
+ * wbase = SOD^2 dtheta 2pi/range (FdkAffineCoeff,
+ * src/global/YkFdkKernelTypes.hpp:35-55), computed in FP64. */
+yk_status yk_geometry_fdk_coefficients(const yk_view* views, int n_views, int nu,
+    int nv, float scan_range_rad, float* out);
+
+/* Ramp-filter frequency weights w[0..N/2] with 1/N baked in
+ * (CreateFilterKernelFromFFT::build_weights, src/Filter/YkCreateFilterKernelFFT.hpp:114-258,
+ * AnalyticFreq source).  N must be a power of two. */
+yk_status yk_filter_weights(const yk_filter* filter, int padded_n, float du_mm,
+    float* out);
+/* smallest power of two >= 2 nu (.../FDK/kernels/YkFDKFilterHelpers.cuh:29-35) */
+int yk_filter_padded_n(int nu);
+
+/* ------------------------------------------------------------------------ */
+/* FDK (FdkPipeline, src/Reconstruction/Analytic/Circular/Flat/FDK/YkFdkPipeline.hpp) */
+/* ------------------------------------------------------------------------ */
+/* Plan for a complete view sequence (prepareWithGeometry, YkFdkPipeline.hpp:344-366). */
+yk_status yk_fdk_create(yk_ctx* ctx, const yk_fdk_desc* desc, const yk_view* views,
+    int n_views, int nu, int nv, const yk_volume_grid* volume, yk_fdk** out);
+/* Plan from a regular scan (prepare(SSystemConfig...), YkFdkPipeline.hpp:247-272);
+ * scan range, start angle, direction and Parker come from the scan. */
+yk_status yk_fdk_create_system(yk_ctx* ctx, const yk_fdk_desc* desc,
+    const yk_system* system, yk_fdk** out);
+yk_status yk_fdk_destroy(yk_fdk* fdk);
+
+/* Streaming batch (Session::execute FDK branch, YkReconstructionApi.cpp:353-370,
+ * YkSession.cpp:693-728): view_offset must equal the views received so far,
+ * clear_output is only allowed on the first batch, capacities are in floats.
+ * The projection may be Host or Device; the volume may be Host or Device.  A
+ * Host volume is written once the last view has been received. */
+yk_status yk_fdk_execute(yk_fdk* fdk, const float* projection, int projection_location,
+    size_t projection_capacity, int view_offset, int view_count, float* volume,
+    int volume_location, size_t volume_capacity, int clear_output);
+yk_status yk_fdk_reset(yk_fdk* fdk);
+int yk_fdk_received_views(const yk_fdk* fdk);
+int yk_fdk_complete(const yk_fdk* fdk);
+
+/* Stage entry points on device buffers.
+ * Cosine preweight [+ Parker] + ramp filter of views [first, first+count):
+ * d_projection holds those views, d_filtered receives them in `storage`
+ * (float / half / bfloat16), d_row_scales receives one float per detector row
+ * for half/bfloat16 (value = stored * scale) and may be NULL for float. */
+yk_status yk_fdk_filter(yk_fdk* fdk, const float* d_projection, int first_view,
+    int count, void* d_filtered, int storage, float* d_row_scales);
+/* Voxel-driven FDK backprojection of filtered views [first, first+count) into
+ * d_volume ([z][y][x]); accumulate = 0 overwrites. */
+yk_status yk_fdk_backproject(yk_fdk* fdk, const void* d_filtered, int storage,
+    const float* d_row_scales, int first_view, int count, float* d_volume,
+    int accumulate);
+/* Introspection for tests: padded row length, weights (N/2+1 floats),
+ * derived geometry (20 floats/view), coefficients (17 floats/view). */
+int yk_fdk_padded_n(const yk_fdk* fdk);
+yk_status yk_fdk_get_weights(const yk_fdk* fdk, float* out);
+yk_status yk_fdk_get_derived(const yk_fdk* fdk, float* out);
+yk_status yk_fdk_get_coefficients(const yk_fdk* fdk, float* out);
+
+/* ------------------------------------------------------------------------ */
+/* projection operators (IForwardOperator / IBackOperator,                   */
+/* src/common/YkProjectionOperators.hpp:24-40)                               */
+/* ------------------------------------------------------------------------ */
+yk_status yk_projector_create(yk_ctx* ctx, const yk_view* views, int n_views,
+    int nu, int nv, const yk_volume_grid* volume, float scan_range_rad,
+    yk_projector** out);
+yk_status yk_projector_destroy(yk_projector* projector);
+/* d_projection holds views [first, first+count); accumulate = 0 overwrites */
+yk_status yk_project(yk_projector* projector, int forward_kind, const float* d_volume,
+    float* d_projection, int first_view, int count, int accumulate);
+yk_status yk_backproject(yk_projector* projector, int back_kind, const float* d_projection,
+    float* d_volume, int first_view, int count, int accumulate);
+
+/* ------------------------------------------------------------------------ */
+/* iterative callers and vector helpers                                      */
+/* ------------------------------------------------------------------------ */
+/* CGLS (doc/algorithms.md:269; CglsRobustBackend::run,
+ * src/Reconstruction/Iterative/Flat/YkCglsBackends.hpp:282-415), x in/out.
+ * residual_norms (optional, iterations+1 doubles) receives ||b - A x_k||. */
+yk_status yk_cgls(yk_projector* projector, int forward_kind, int back_kind,
+    const float* d_b, float* d_x, int iterations, double* residual_norms);
+/* SART with view-by-view updates (AlgebraicReconstructorEx,
+ * src/Reconstruction/Iterative/Flat/YkAlgebraicReconstructorEx.hpp:137-180) */
+yk_status yk_sart(yk_projector* projector, int forward_kind, int back_kind,
+    const float* d_b, float* d_x, int iterations, float relaxation,
+    int nonnegative);
+/* smoothed TV value and gradient (YkTvRegularizationKernels.cu:17-73) */
+yk_status yk_tv_gradient(yk_ctx* ctx, const yk_volume_grid* volume, const float* d_x,
+    float epsilon, float* d_gradient, double* value);
+/* FP64-accumulated dot product of two device float arrays */
+yk_status yk_dot(yk_ctx* ctx, const float* d_a, const float* d_b, size_t n, double* out);
+/* Point-sampled ellipsoid phantom into a device volume.  Each ellipsoid is 16
+ * floats: centre[3], semi_axes[3], R[9] (body = R (p - centre)), value. */
+yk_status yk_phantom_ellipsoids(yk_ctx* ctx, const float* ellipsoids, int count,
+    const yk_volume_grid* volume, int clamp_nonnegative, float* d_volume);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,316 @@
+"""ctypes front end of the FP64 CPU oracle (``libyk_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py``.  The product package never imports
+this module.  See ``yk_oracle.h`` for the reference citations of each routine.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libyk_oracle.so")
+
+VIEW_STRIDE = 13
+DERIVED_STRIDE = 20
+COEFF_STRIDE = 17
+ELLIPSOID_STRIDE = 16
+
+# derived-record offsets
+D_THETA, D_DTHETA, D_FSCALE, D_SOD, D_SDD_RADIAL, D_SDD_PLANE = 0, 1, 2, 3, 4, 5
+D_OFFU, D_OFFV, D_DU, D_DV, D_RADIAL, D_NORMAL = 6, 7, 8, 9, 10, 13
+# coefficient offsets
+C_CU, C_CV, C_CD, C_CR, C_WBASE = 0, 4, 8, 12, 16
+
+FILTERS = {"none": 0, "ramlak": 1, "shepplogan": 2, "cosine": 3, "hann": 4,
+           "hamming": 5, "blackman": 6, "butterworth": 7, "kaiser": 8, "tukey": 9}
+
+
+class Volume(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("vx", ctypes.c_double), ("vy", ctypes.c_double), ("vz", ctypes.c_double),
+                ("cx", ctypes.c_double), ("cy", ctypes.c_double), ("cz", ctypes.c_double)]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc -O2 -fopenmp)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "liboracle"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
+        ip = ctypes.POINTER(ctypes.c_int)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        vp = ctypes.POINTER(Volume)
+        c_int, c_double = ctypes.c_int, ctypes.c_double
+        sig = {
+            "yko_build_circular_flat_views": (c_int, [c_int, c_int, c_double, c_int, c_double, c_double,
+                                                      c_double, dp, c_int, c_int, c_double, c_double, dp,
+                                                      c_double, c_double, c_double, dp]),
+            "yko_derive": (c_int, [dp, c_int, c_int, c_int, c_double, dp]),
+            "yko_fdk_coeffs": (c_int, [dp, dp, c_int, dp]),
+            "yko_preweight": (None, [dp, dp, c_int, c_int, c_int, dp, dp]),
+            "yko_parker": (None, [dp, dp, c_int, c_int, c_int, c_double, c_int, c_double, dp]),
+            "yko_padded_n": (c_int, [c_int]),
+            "yko_start_u": (c_int, [c_int, c_int, c_double]),
+            "yko_bessel_i0_as": (c_double, [c_double]),
+            "yko_bessel_i0_series": (c_double, [c_double]),
+            "yko_window": (c_double, [c_int, c_double, c_double, c_double, c_double, c_int]),
+            "yko_filter_weights": (c_int, [c_int, c_int, c_double, c_double, c_double, c_double,
+                                           c_double, c_int, c_double, dp]),
+            "yko_filter_kernel_from_weights": (None, [dp, c_int, dp]),
+            "yko_filter_rows": (None, [dp, c_int, c_int, c_int, c_int, dp, ip, c_int, dp]),
+            "yko_fft": (None, [dp, dp, c_int, c_int]),
+            "yko_bilinear": (c_double, [dp, c_int, c_int, c_double, c_double]),
+            "yko_bp_fdk": (None, [dp, c_int, dp, c_int, c_int, vp, i64p, ctypes.c_int64, dp]),
+            "yko_main_axis": (c_int, [dp, c_int, c_int]),
+            "yko_fp_joseph": (None, [dp, c_int, c_int, c_int, vp, fp, dp]),
+            "yko_bp_joseph_adjoint": (None, [dp, c_int, c_int, c_int, vp, dp, dp]),
+            "yko_fp_siddon": (None, [dp, c_int, c_int, c_int, vp, fp, dp]),
+            "yko_bp_siddon_adjoint": (None, [dp, c_int, c_int, c_int, vp, dp, dp]),
+            "yko_siddon_ray": (c_double, [dp, dp, vp, fp]),
+            "yko_phantom_ellipsoids": (None, [dp, c_int, vp, c_int, fp]),
+            "yko_ellipsoid_line_integrals": (None, [dp, c_int, dp, c_int, c_int, c_int, dp]),
+            "yko_tv_value_grad": (None, [dp, vp, c_double, dp, dp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def volume(nx, ny, nz, vx, vy=None, vz=None, center=(0.0, 0.0, 0.0)) -> Volume:
+    vy = vx if vy is None else vy
+    vz = vx if vz is None else vz
+    return Volume(nx, ny, nz, vx, vy, vz, *center)
+
+
+def build_views(total_views, views_per_turn, sid, sdd, channels, rows, du, dv,
+                start_angle=0.0, direction=1, z_mm=0.0, source_offset=(0, 0, 0),
+                offset_unv=(0, 0, 0), tilt=(0, 0, 0)) -> np.ndarray:
+    out = np.zeros((total_views, VIEW_STRIDE))
+    so = _d(source_offset)
+    off = _d(offset_unv)
+    rc = lib().yko_build_circular_flat_views(total_views, views_per_turn, start_angle, direction,
+                                             sid, sdd, z_mm, _dp(so), channels, rows, du, dv,
+                                             _dp(off), tilt[0], tilt[1], tilt[2], _dp(out))
+    if rc != 0:
+        raise ValueError("invalid circular scan")
+    return out
+
+
+def views_from_float(views_f32) -> np.ndarray:
+    """(n, 20) float32 product views (5 x float4) -> (n, 13) float64 oracle views."""
+    v = np.asarray(views_f32, dtype=np.float32).reshape(-1, 5, 4).astype(np.float64)
+    out = np.zeros((v.shape[0], VIEW_STRIDE))
+    out[:, 0:3] = v[:, 0, :3]
+    out[:, 3:6] = v[:, 1, :3]
+    out[:, 6:9] = v[:, 2, :3]
+    out[:, 9:12] = v[:, 3, :3]
+    out[:, 12] = v[:, 4, 0]
+    return out
+
+
+def derive(views, nu, nv, scan_range) -> np.ndarray:
+    views = _d(views)
+    out = np.zeros((views.shape[0], DERIVED_STRIDE))
+    rc = lib().yko_derive(_dp(views), views.shape[0], nu, nv, scan_range, _dp(out))
+    if rc != 0:
+        raise ValueError("degenerate geometry")
+    return out
+
+
+def coeffs(views, derived) -> np.ndarray:
+    views, derived = _d(views), _d(derived)
+    out = np.zeros((views.shape[0], COEFF_STRIDE))
+    lib().yko_fdk_coeffs(_dp(views), _dp(derived), views.shape[0], _dp(out))
+    return out
+
+
+def preweight(views, derived, proj) -> np.ndarray:
+    views, derived, proj = _d(views), _d(derived), _d(proj)
+    n, nv, nu = proj.shape
+    out = np.zeros_like(proj)
+    lib().yko_preweight(_dp(views), _dp(derived), n, nu, nv, _dp(proj), _dp(out))
+    return out
+
+
+def parker(views, derived, proj, start_angle, direction, scan_range) -> np.ndarray:
+    views, derived = _d(views), _d(derived)
+    data = _d(proj).copy()
+    n, nv, nu = data.shape
+    lib().yko_parker(_dp(views), _dp(derived), n, nu, nv, start_angle, direction, scan_range, _dp(data))
+    return data
+
+
+def padded_n(nu):
+    return lib().yko_padded_n(nu)
+
+
+def start_u(nu, n, offset_u_pix):
+    return lib().yko_start_u(nu, n, offset_u_pix)
+
+
+def filter_weights(kind, n, du, cutoff=0.5, gain=1.0, order=2.0, beta=8.6, tukey_alpha=0.5,
+                   source=0) -> np.ndarray:
+    k = FILTERS[kind] if isinstance(kind, str) else int(kind)
+    out = np.zeros(n // 2 + 1)
+    lib().yko_filter_weights(k, source, cutoff, gain, order, beta, tukey_alpha, n, du, _dp(out))
+    return out
+
+
+def filter_kernel(weights, n) -> np.ndarray:
+    w = _d(weights)
+    h = np.zeros(n)
+    lib().yko_filter_kernel_from_weights(_dp(w), n, _dp(h))
+    return h
+
+
+def filter_rows(proj, weights, n, start_u_per_view, method=1) -> np.ndarray:
+    proj = _d(proj)
+    nviews, nv, nu = proj.shape
+    su = np.ascontiguousarray(start_u_per_view, dtype=np.int32)
+    w = _d(weights)
+    out = np.zeros_like(proj)
+    lib().yko_filter_rows(_dp(proj), nviews, nv, nu, n, _dp(w),
+                          su.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), method, _dp(out))
+    return out
+
+
+def fft(re, im, inverse=False):
+    re, im = _d(re).copy(), _d(im).copy()
+    lib().yko_fft(_dp(re), _dp(im), re.size, 1 if inverse else 0)
+    return re, im
+
+
+def bilinear(plane, u, v) -> float:
+    plane = _d(plane)
+    nv, nu = plane.shape
+    return lib().yko_bilinear(_dp(plane), nu, nv, u, v)
+
+
+def bp_fdk(coeffs_, filtered, vol: Volume, voxel_index=None) -> np.ndarray:
+    c = _d(coeffs_)
+    f = _d(filtered)
+    n, nv, nu = f.shape
+    if voxel_index is None:
+        out = np.zeros((vol.nz, vol.ny, vol.nx))
+        lib().yko_bp_fdk(_dp(c), n, _dp(f), nu, nv, ctypes.byref(vol), None, 0, _dp(out))
+    else:
+        idx = np.ascontiguousarray(voxel_index, dtype=np.int64)
+        out = np.zeros(idx.size)
+        lib().yko_bp_fdk(_dp(c), n, _dp(f), nu, nv, ctypes.byref(vol),
+                         idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), idx.size, _dp(out))
+    return out
+
+
+def main_axis(view, nu, nv) -> int:
+    return lib().yko_main_axis(_dp(_d(view)), nu, nv)
+
+
+def fp_joseph(views, nu, nv, vol: Volume, volume_f32) -> np.ndarray:
+    views = _d(views)
+    x = _f(volume_f32)
+    out = np.zeros((views.shape[0], nv, nu))
+    lib().yko_fp_joseph(_dp(views), views.shape[0], nu, nv, ctypes.byref(vol), _fp(x), _dp(out))
+    return out
+
+
+def bp_joseph_adjoint(views, nu, nv, vol: Volume, proj) -> np.ndarray:
+    views, p = _d(views), _d(proj)
+    out = np.zeros((vol.nz, vol.ny, vol.nx))
+    lib().yko_bp_joseph_adjoint(_dp(views), views.shape[0], nu, nv, ctypes.byref(vol), _dp(p), _dp(out))
+    return out
+
+
+def fp_siddon(views, nu, nv, vol: Volume, volume_f32) -> np.ndarray:
+    views = _d(views)
+    x = _f(volume_f32)
+    out = np.zeros((views.shape[0], nv, nu))
+    lib().yko_fp_siddon(_dp(views), views.shape[0], nu, nv, ctypes.byref(vol), _fp(x), _dp(out))
+    return out
+
+
+def bp_siddon_adjoint(views, nu, nv, vol: Volume, proj) -> np.ndarray:
+    views, p = _d(views), _d(proj)
+    out = np.zeros((vol.nz, vol.ny, vol.nx))
+    lib().yko_bp_siddon_adjoint(_dp(views), views.shape[0], nu, nv, ctypes.byref(vol), _dp(p), _dp(out))
+    return out
+
+
+def siddon_ray(src, dst, vol: Volume, volume_f32) -> float:
+    return lib().yko_siddon_ray(_dp(_d(src)), _dp(_d(dst)), ctypes.byref(vol), _fp(_f(volume_f32)))
+
+
+def phantom(ellipsoids, vol: Volume, clamp=True) -> np.ndarray:
+    e = _d(ellipsoids).reshape(-1, ELLIPSOID_STRIDE)
+    out = np.zeros((vol.nz, vol.ny, vol.nx), dtype=np.float32)
+    lib().yko_phantom_ellipsoids(_dp(e), e.shape[0], ctypes.byref(vol), 1 if clamp else 0, _fp(out))
+    return out
+
+
+def ellipsoid_line_integrals(ellipsoids, views, nu, nv) -> np.ndarray:
+    e = _d(ellipsoids).reshape(-1, ELLIPSOID_STRIDE)
+    views = _d(views)
+    out = np.zeros((views.shape[0], nv, nu))
+    lib().yko_ellipsoid_line_integrals(_dp(e), e.shape[0], _dp(views), views.shape[0], nu, nv, _dp(out))
+    return out
+
+
+def tv_value_grad(x, vol: Volume, eps):
+    x = _d(x)
+    g = np.zeros_like(x)
+    val = ctypes.c_double(0.0)
+    lib().yko_tv_value_grad(_dp(x), ctypes.byref(vol), eps, ctypes.byref(val), _dp(g))
+    return val.value, g
+
+
+def fdk(views, proj, vol: Volume, filt="ramlak", scan_range=None, cutoff=0.5, gain=1.0,
+        parker_start=None, direction=1, voxel_index=None, method=1):
+    """Whole FDK chain in FP64: derive, preweight, [Parker], filter, backproject."""
+    views = _d(views)
+    n, nv, nu = proj.shape
+    if scan_range is None:
+        scan_range = 2.0 * np.pi
+    d = derive(views, nu, nv, scan_range)
+    c = coeffs(views, d)
+    q = preweight(views, d, proj)
+    if parker_start is not None:
+        q = parker(views, d, q, parker_start, direction, scan_range)
+    N = padded_n(nu)
+    w = filter_weights(filt, N, d[0, D_DU], cutoff=cutoff, gain=gain)
+    su = [start_u(nu, N, d[i, D_OFFU]) for i in range(n)]
+    f = filter_rows(q, w, N, su, method=method)
+    return bp_fdk(c, f, vol, voxel_index)
