@@ -1,0 +1,180 @@
+"""FDK backprojection throughput on one MI355X.
+
+Workload (the reference's 512^3 x 720 configuration): 512^3 voxels at 0.25 mm,
+SOD 600 mm, SDD 1100 mm, 720 views over 360 degrees, flat detector 1024 x 768
+at 0.3 mm with a 2.5 px (0.75 mm) horizontal offset, Hann-windowed ramp.  The
+phantom is 64 random ellipsoids (semi-axes 5-40 mm, densities 0-0.05 / mm,
+seed 2), voxelized and forward-projected on the GPU by the Joseph projector.
+
+A step is one backprojection of all 720 views into the 512^3 volume from
+filtered projections resident in HBM.  The headline value is the GUPS of the
+FP16-stored path (half rows with per-row scales, FP32 + Kahan accumulation);
+the FP32 path is timed beside it.  Accuracy of both is measured against the
+FP64 oracle on 8 evenly spaced z-slices, whose oracle backprojection is also
+the timed CPU baseline.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+METRIC = "FDK backprojection GUPS at 512³×720; FP16-stored vs FP32 at ≤1e-3 rel-RMSE"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+FLOP_PER_UPDATE = {"f32": 25, "f16_kahan": 29}  # algorithmic FLOPs per voxel-view update
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle accuracy/CPU leg")
+    ap.add_argument("--size", type=int, default=512, help="volume edge (default 512)")
+    ap.add_argument("--views", type=int, default=720)
+    return ap.parse_args()
+
+
+def time_launches(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
+def oracle_check(views32, proj_cpu, geom, vols, args):
+    """FP64 oracle FDK on 8 evenly spaced z-slices; returns accuracy per
+    volume and the timed oracle backprojection (CPU baseline)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import yk_oracle as O
+    ov = O.views_from_float(views32)
+    nu, nv = geom.channels, geom.rows
+    d = O.derive(ov, nu, nv, geom.scan_range)
+    c = O.coeffs(ov, d)
+    q = O.preweight(ov, d, proj_cpu.astype(np.float64))
+    n = O.padded_n(nu)
+    w = O.filter_weights("hann", n, float(np.float32(d[0, O.D_DU])), cutoff=0.5)
+    su = [O.start_u(nu, n, d[i, O.D_OFFU]) for i in range(len(ov))]
+    f = O.filter_rows(q, w, n, su, method=1)
+    del q
+    nx, ny, nz = geom.volume
+    slices = np.linspace(0, nz - 1, 8).round().astype(np.int64)
+    idx = (slices[:, None] * (nx * ny) + np.arange(nx * ny, dtype=np.int64)[None, :]).ravel()
+    vol = O.volume(nx, ny, nz, *geom.voxel_mm, center=geom.center_mm)
+    t0 = time.perf_counter()
+    ref = O.bp_fdk(c, f, vol, idx)
+    cpu_s = time.perf_counter() - t0
+    acc = {}
+    for name, v in vols.items():
+        got = v.reshape(-1)[idx].astype(np.float64)
+        acc[name] = {"rel_rmse": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+                     "max_abs": float(np.abs(got - ref).max()),
+                     "ref_max_abs": float(np.abs(ref).max())}
+    updates = idx.size * len(ov)
+    cpu = {"value": updates / cpu_s / 1e9, "unit": "GUPS", "cores": O.num_threads(),
+           "kind": "port",
+           "sample": f"FP64 oracle backprojection of {len(slices)} z-slices "
+                     f"({idx.size} voxels) x {len(ov)} views = {updates:.3e} updates "
+                     f"in {cpu_s:.1f} s"}
+    return acc, cpu
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    sys.path.insert(0, REPO)
+    import fdk_cuda_impl_amd as yk
+    from fdk_cuda_impl_amd import phantoms
+
+    torch.cuda.set_device(0)
+    s = args.size
+    vox = 0.25 * 512 / s
+    geom = yk.ScanGeometry(args.views, args.views, 600.0, 1100.0, 1024, 768, 0.3, 0.3,
+                           volume=(s, s, s), voxel_mm=(vox, vox, vox), offset_unv_mm=(0.75, 0.0, 0.0))
+    ctx = yk.Context(0)
+    stream = torch.cuda.current_stream()
+    grid = geom.grid()
+    views = geom.views()
+    ell = phantoms.random_ellipsoids(2, 64, semi_axes=(5.0, 40.0), density=(0.0, 0.05),
+                                     center_radius=50.0)
+    truth = ctx.phantom(ell, grid, clamp=False)
+    P = yk.Projector(ctx, views, geom.nu, geom.nv, grid)
+    t0 = time.perf_counter()
+    proj = P.forward(truth, kind="joseph")
+    torch.cuda.synchronize()
+    fp_s = time.perf_counter() - t0
+    del truth
+    print(f"[bench] projections ready ({fp_s:.2f} s Joseph FP)", file=sys.stderr, flush=True)
+
+    plan32 = yk.FdkPlan.from_geometry(ctx, geom, filter="hann")
+    plan16 = yk.FdkPlan.from_geometry(ctx, geom, filter="hann", storage="f16", kahan=True)
+    ctx.set_synchronous(False)
+    f32, _ = plan32.filter(proj)
+    f16, s16 = plan16.filter(proj)
+    v32 = torch.empty(plan32.volume_shape, device="cuda")
+    v16 = torch.empty(plan16.volume_shape, device="cuda")
+    filt_ms = time_launches(lambda: plan16.filter(proj, out=f16, scales=s16), 3, 1, stream)
+    ms32 = time_launches(lambda: plan32.backproject(f32, None, volume=v32), args.steps, args.warmup, stream)
+    print(f"[bench] fp32 backprojection {ms32:.2f} ms", file=sys.stderr, flush=True)
+    ms16 = time_launches(lambda: plan16.backproject(f16, s16, volume=v16), args.steps, args.warmup, stream)
+    print(f"[bench] fp16 backprojection {ms16:.2f} ms", file=sys.stderr, flush=True)
+    ctx.set_synchronous(True)
+
+    nvox = s * s * s
+    updates = nvox * args.views
+    gups16 = updates / (ms16 * 1e-3) / 1e9
+    gups32 = updates / (ms32 * 1e-3) / 1e9
+    achieved = updates * FLOP_PER_UPDATE["f16_kahan"] / (ms16 * 1e-3) / 1e12
+    alg_bytes16 = args.views * geom.nu * geom.nv * 2 + args.views * geom.nv * 4 + nvox * 4
+
+    result = {
+        "metric": METRIC, "value": round(gups16, 2), "unit": "GUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms16, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16",
+        "data": "synthetic (64 seeded random ellipsoids, GPU Joseph projections)",
+        "config": {"workload": f"FDK backprojection {s}^3 voxels @ {vox} mm x {args.views} views, "
+                               f"1024x768 @ 0.3 mm detector (+2.5 px offset), SOD 600 / SDD 1100, Hann",
+                   "storage": "fp16 rows + per-row fp32 scales, fp32 Kahan accumulation",
+                   "views_per_launch": args.views},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "note": f"VALU/LDS-bound kernel; {FLOP_PER_UPDATE['f16_kahan']} algorithmic FLOP per "
+                             f"update; HBM {alg_bytes16 / (ms16 * 1e-3) / 1e9:.0f} GB/s algorithmic"},
+        "fp32_gups": round(gups32, 2), "fp32_ms_per_step": round(ms32, 3),
+        "fp16_speedup_vs_fp32": round(ms32 / ms16, 3),
+        "filter_ms_fp16": round(filt_ms, 3),
+        "joseph_fp_s": round(fp_s, 3),
+    }
+    if not args.no_cpu:
+        print("[bench] FP64 oracle check", file=sys.stderr, flush=True)
+        acc, cpu = oracle_check(views, proj.cpu().numpy(), geom,
+                                {"f16": v16.cpu().numpy(), "f32": v32.cpu().numpy()}, args)
+        result["accuracy_vs_fp64"] = acc
+        result["cpu_baseline"] = cpu
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

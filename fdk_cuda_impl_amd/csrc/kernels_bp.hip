@@ -40,7 +40,7 @@ namespace {
 constexpr int kBX = 16;
 constexpr int kBY = 16;
 constexpr int kThreads = kBX * kBY;
-constexpr int kKahanViews = 8;
+constexpr int kKahanViews = 16;
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
@@ -122,80 +122,241 @@ __global__ void max_scale_kernel(const float* __restrict__ s, int n, int* __rest
 }
 
 // ---------------------------------------------------------------------------
-// staging: fill LDS quads for tile (u0, v0, qw, qh) of one view
+// staging: the quads of tile (u0, v0, qw, qh) of one view.  Lanes take
+// consecutive quad columns; each thread owns a run of quad rows in one column
+// and slides a two-row window down it.  For tiles up to kMaxRows rows per
+// thread the texels of the NEXT view are fetched into registers before the
+// current view is accumulated, so global latency hides behind the LDS work.
 // ---------------------------------------------------------------------------
-template <int STORAGE>
-__device__ __forceinline__ float texel(const void* __restrict__ proj, size_t off)
+constexpr int kMaxRows = 16;  // texel rows a thread can hold (quad rows + 1)
+
+struct StageMap {
+    int col;      // quad column
+    int r0;       // first quad row
+    int rows;     // quad rows written by this thread (0: idle)
+    int rows_per; // quad rows per thread (uniform)
+    bool fast;    // fits the register prefetch
+};
+
+__device__ __forceinline__ StageMap stage_map(short4 t)
 {
-    if constexpr (STORAGE == YK_F32) return static_cast<const float*>(proj)[off];
-    else if constexpr (STORAGE == YK_F16) return __half2float(static_cast<const __half*>(proj)[off]);
-    else return __bfloat162float(static_cast<const __hip_bfloat16*>(proj)[off]);
+    StageMap m;
+    const int qw = t.z, qh = t.w;
+    const int span = min(qw, kThreads);
+    const int chunks = kThreads / span;
+    const int rows_per = (qh + chunks - 1) / chunks;
+    const int rc = int(threadIdx.x) / span;
+    m.col = int(threadIdx.x) % span;
+    m.r0 = rc * rows_per;
+    m.rows = rc < chunks ? max(0, min(rows_per, qh - m.r0)) : 0;
+    m.rows_per = rows_per;
+    m.fast = qw <= kThreads && rows_per + 1 <= kMaxRows;
+    return m;
 }
 
-template <int STORAGE>
-__device__ __forceinline__ void load_pair(const void* __restrict__ view_base,
-    const float* __restrict__ view_scales_row, int nu, int nv, int row, int col, bool inside,
-    float inv_g, float& a, float& b)
+__device__ __forceinline__ bool tile_inside(short4 t, int nu, int nv)
 {
-    if (inside || (row >= 0 && row < nv)) {
-        const size_t off = (size_t)row * nu;
-        float s = 1.f;
-        if constexpr (STORAGE != YK_F32) s = view_scales_row[row] * inv_g;
+    return t.x >= 0 && t.y >= 0 && t.x + t.z + 1 <= nu && t.y + t.w + 1 <= nv;
+}
+
+// raw texel pair (col, col+1) of one detector row: float2 for float storage,
+// two 16-bit patterns packed lo/hi for half / bfloat16; zero outside
+template <int STORAGE>
+struct Pair;
+template <>
+struct Pair<YK_F32> { using T = float2; };
+template <>
+struct Pair<YK_F16> { using T = unsigned; };
+template <>
+struct Pair<YK_BF16> { using T = unsigned; };
+
+template <int STORAGE>
+__device__ __forceinline__ typename Pair<STORAGE>::T fetch_pair(const void* __restrict__ base, int nu,
+    int nv, int row, int col, bool inside)
+{
+    if constexpr (STORAGE == YK_F32) {
+        const float* p = static_cast<const float*>(base);
         if (inside) {
-            a = texel<STORAGE>(view_base, off + col) * s;
-            b = texel<STORAGE>(view_base, off + col + 1) * s;
-        } else {
-            a = (col >= 0 && col < nu) ? texel<STORAGE>(view_base, off + col) * s : 0.f;
-            b = (col + 1 >= 0 && col + 1 < nu) ? texel<STORAGE>(view_base, off + col + 1) * s : 0.f;
+            const float* r = p + (size_t)row * nu + col;
+            return make_float2(r[0], r[1]);
         }
+        float2 v = make_float2(0.f, 0.f);
+        if (row >= 0 && row < nv) {
+            const float* r = p + (size_t)row * nu;
+            if (col >= 0 && col < nu) v.x = r[col];
+            if (col + 1 >= 0 && col + 1 < nu) v.y = r[col + 1];
+        }
+        return v;
     } else {
-        a = 0.f;
-        b = 0.f;
+        const unsigned short* p = static_cast<const unsigned short*>(base);
+        unsigned lo = 0, hi = 0;
+        if (inside) {
+            const unsigned short* r = p + (size_t)row * nu + col;
+            lo = r[0];
+            hi = r[1];
+        } else if (row >= 0 && row < nv) {
+            const unsigned short* r = p + (size_t)row * nu;
+            if (col >= 0 && col < nu) lo = r[col];
+            if (col + 1 >= 0 && col + 1 < nu) hi = r[col + 1];
+        }
+        return lo | (hi << 16);
+    }
+}
+
+// row normalizer: stored * scale / global scale (an exact power of two)
+__device__ __forceinline__ float row_factor(const float* __restrict__ scales, int nv, int row, float inv_g)
+{
+    return (row >= 0 && row < nv) ? scales[row] * inv_g : 0.f;
+}
+
+// normalized half pair of a raw pair
+template <int STORAGE>
+__device__ __forceinline__ half2_t to_half2(unsigned raw, float f)
+{
+    if constexpr (STORAGE == YK_F16) {
+        const half2_t h = __builtin_bit_cast(half2_t, raw);
+        const _Float16 fh = (_Float16)f;  // power of two: exact when representable
+        const half2_t ff = { fh, fh };
+        return h * ff;
+    } else {
+        const float a = __uint_as_float(raw << 16) * f;
+        const float b = __uint_as_float(raw & 0xffff0000u) * f;
+        const half2_t r = { (_Float16)a, (_Float16)b };
+        return r;
     }
 }
 
 template <int STORAGE>
-__device__ __forceinline__ void store_quad(void* lds, int index, float t0, float t1, float b0, float b1)
+__device__ __forceinline__ void write_quad(void* lds, int index, const typename Pair<STORAGE>::T& top,
+    const typename Pair<STORAGE>::T& bot, float ftop, float fbot)
 {
     if constexpr (STORAGE == YK_F32) {
-        static_cast<float4*>(lds)[index] = make_float4(t0, t1, b0 - t0, b1 - t1);
+        static_cast<float4*>(lds)[index] = make_float4(top.x, top.y, bot.x - top.x, bot.y - top.y);
     } else {
-        const half2_t h = { (_Float16)t0, (_Float16)t1 };
-        const half2_t d = { (_Float16)(b0 - t0), (_Float16)(b1 - t1) };
+        const half2_t a = to_half2<STORAGE>(top, ftop);
+        const half2_t b = to_half2<STORAGE>(bot, fbot);
+        const half2_t d = b - a;
         uint2 q;
-        q.x = __builtin_bit_cast(unsigned, h);
+        q.x = __builtin_bit_cast(unsigned, a);
         q.y = __builtin_bit_cast(unsigned, d);
         static_cast<uint2*>(lds)[index] = q;
     }
 }
 
+// direct staging (no register prefetch), used for very tall or wide tiles
 template <int STORAGE>
-__device__ __forceinline__ void stage_tile(void* lds, const void* __restrict__ view_base,
-    const float* __restrict__ scales_row, int nu, int nv, short4 t, float inv_g)
+__device__ void stage_direct(void* lds, const void* __restrict__ base, const float* __restrict__ scales,
+    int nu, int nv, short4 t, float inv_g)
 {
     const int qw = t.z, qh = t.w;
-    const bool inside = t.x >= 0 && t.y >= 0 && t.x + qw + 1 <= nu && t.y + qh + 1 <= nv;
-    // lanes walk down columns; a pass covers `span` columns with `chunks`
-    // row ranges each, so every thread slides a 2-row window over its range
+    const bool inside = tile_inside(t, nu, nv);
     const int span = min(qw, kThreads);
     const int chunks = kThreads / span;
     const int rows_per = (qh + chunks - 1) / chunks;
-    const int tid = threadIdx.x;
-    const int rc = tid / span;
+    const int rc = int(threadIdx.x) / span;
     if (rc >= chunks) return;
     const int r0 = rc * rows_per;
     const int r1 = min(r0 + rows_per, qh);
-    if (r0 >= r1) return;
-    for (int c = tid % span; c < qw; c += span) {
-        const int col = t.x + c;
-        float a0, a1;
-        load_pair<STORAGE>(view_base, scales_row, nu, nv, t.y + r0, col, inside, inv_g, a0, a1);
+    for (int c = int(threadIdx.x) % span; c < qw; c += span) {
+        if (r0 >= r1) break;
+        auto top = fetch_pair<STORAGE>(base, nu, nv, t.y + r0, t.x + c, inside);
+        float ft = 1.f;
+        if constexpr (STORAGE != YK_F32) ft = row_factor(scales, nv, t.y + r0, inv_g);
         for (int r = r0; r < r1; ++r) {
-            float b0, b1;
-            load_pair<STORAGE>(view_base, scales_row, nu, nv, t.y + r + 1, col, inside, inv_g, b0, b1);
-            store_quad<STORAGE>(lds, r * qw + c, a0, a1, b0, b1);
-            a0 = b0;
-            a1 = b1;
+            auto bot = fetch_pair<STORAGE>(base, nu, nv, t.y + r + 1, t.x + c, inside);
+            float fb = 1.f;
+            if constexpr (STORAGE != YK_F32) fb = row_factor(scales, nv, t.y + r + 1, inv_g);
+            write_quad<STORAGE>(lds, r * qw + c, top, bot, ft, fb);
+            top = bot;
+            ft = fb;
+        }
+    }
+}
+
+// Register prefetch: loads only, no use of the loaded values, so the compiler
+// can leave every load in flight until commit.  Rows and columns are clamped
+// into the detector so each load is in bounds; commit zeroes what lies
+// outside.  Requires nu >= 2.
+template <int STORAGE>
+struct Prefetch {
+    typename Pair<STORAGE>::T v[kMaxRows];
+    float f[STORAGE == YK_F32 ? 1 : kMaxRows];
+};
+
+template <int STORAGE>
+__device__ __forceinline__ typename Pair<STORAGE>::T raw_pair(const void* __restrict__ base, int nu,
+    int row, int col)
+{
+    if constexpr (STORAGE == YK_F32) {
+        const float* r = static_cast<const float*>(base) + (size_t)row * nu + col;
+        return make_float2(r[0], r[1]);
+    } else {
+        // two adjacent 16-bit texels as one (possibly 2-byte aligned) dword
+        const unsigned short* r = static_cast<const unsigned short*>(base) + (size_t)row * nu + col;
+        return unsigned(r[0]) | (unsigned(r[1]) << 16);
+    }
+}
+
+template <int STORAGE>
+__device__ __forceinline__ void prefetch_tile(Prefetch<STORAGE>& pf, const StageMap& m,
+    const void* __restrict__ base, const float* __restrict__ scales, int nu, int nv, short4 t)
+{
+    const int cl = min(max(t.x + m.col, 0), nu - 2);
+    const int rbase = t.y + m.r0;
+    const int rows_per = m.rows_per;  // uniform across the workgroup
+#pragma unroll
+    for (int j = 0; j < kMaxRows; ++j) {
+        if (j <= rows_per) {
+            const int row = min(max(rbase + j, 0), nv - 1);
+            pf.v[j] = raw_pair<STORAGE>(base, nu, row, cl);
+            if constexpr (STORAGE != YK_F32) pf.f[j] = scales[row];
+        }
+    }
+}
+
+// shift = requested first column - loaded first column (0 inside the detector)
+template <int STORAGE>
+__device__ __forceinline__ typename Pair<STORAGE>::T fix_pair(typename Pair<STORAGE>::T v, int shift, bool row_ok)
+{
+    if constexpr (STORAGE == YK_F32) {
+        float2 r;
+        r.x = shift == 0 ? v.x : (shift == 1 ? v.y : 0.f);
+        r.y = shift == 0 ? v.y : (shift == -1 ? v.x : 0.f);
+        return row_ok ? r : make_float2(0.f, 0.f);
+    } else {
+        unsigned r = shift == 0 ? v : (shift == 1 ? (v >> 16) : (shift == -1 ? (v << 16) : 0u));
+        return row_ok ? r : 0u;
+    }
+}
+
+template <int STORAGE>
+__device__ __forceinline__ void commit_tile(void* lds, const Prefetch<STORAGE>& pf, const StageMap& m,
+    short4 t, int nu, int nv, float inv_g)
+{
+    if (m.rows <= 0) return;
+    const int col0 = t.x + m.col;
+    const int shift = col0 - min(max(col0, 0), nu - 2);
+    const int rbase = t.y + m.r0;
+    if (tile_inside(t, nu, nv)) {
+#pragma unroll
+        for (int j = 0; j + 1 < kMaxRows; ++j) {
+            if (j < m.rows) {
+                float ft = 1.f, fb = 1.f;
+                if constexpr (STORAGE != YK_F32) { ft = pf.f[j] * inv_g; fb = pf.f[j + 1] * inv_g; }
+                write_quad<STORAGE>(lds, (m.r0 + j) * t.z + m.col, pf.v[j], pf.v[j + 1], ft, fb);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j + 1 < kMaxRows; ++j) {
+            if (j < m.rows) {
+                const int rt = rbase + j, rb = rt + 1;
+                const bool okt = rt >= 0 && rt < nv, okb = rb >= 0 && rb < nv;
+                float ft = 1.f, fb = 1.f;
+                if constexpr (STORAGE != YK_F32) { ft = pf.f[j] * inv_g; fb = pf.f[j + 1] * inv_g; }
+                write_quad<STORAGE>(lds, (m.r0 + j) * t.z + m.col, fix_pair<STORAGE>(pf.v[j], shift, okt),
+                    fix_pair<STORAGE>(pf.v[j + 1], shift, okb), ft, fb);
+            }
         }
     }
 }
@@ -203,18 +364,21 @@ __device__ __forceinline__ void stage_tile(void* lds, const void* __restrict__ v
 // ---------------------------------------------------------------------------
 // per-view accumulation into acc[BZ]
 // ---------------------------------------------------------------------------
+constexpr int kBatch = 8;  // LDS reads issued before their results are consumed
+
 template <int STORAGE, int BZ, bool GENERAL>
 __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& c, short4 t,
-    float X, float Y, float Z0, float vz, float wscale, float (&acc)[BZ])
+    float X, float Y, float Z0, float vz, float (&acc)[BZ])
 {
     const float ul0 = float(t.x), vl0 = float(t.y);
     const int rowq = t.z;
     // distance weight: the radial ray has no z component, so it is z-invariant
     const float depth = c.cr[3] + c.cr[0] * X + c.cr[1] * Y;
-    const float w = c.wbase / (depth * depth) * wscale;
+    const float rd = __builtin_amdgcn_rcpf(depth);
+    const float w = c.wbase * rd * rd;
     if constexpr (!GENERAL) {
         const float den = c.cd[3] + c.cd[0] * X + c.cd[1] * Y;
-        const float r = 1.f / den;
+        const float r = __builtin_amdgcn_rcpf(den);
         const float u = (c.cu[3] + c.cu[0] * X + c.cu[1] * Y) * r - ul0;
         const float v0 = (c.cv[3] + c.cv[0] * X + c.cv[1] * Y + c.cv[2] * Z0) * r - vl0;
         const float dv = c.cv[2] * vz * r;
@@ -225,34 +389,49 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
             const char* base = static_cast<const char*>(lds) + iu * 16;
             const int rowb = rowq * 16;
 #pragma unroll
-            for (int k = 0; k < BZ; ++k) {
-                const float v = fmaf(float(k), dv, v0);
-                const int iv = int(v);
-                const float fv = __builtin_amdgcn_fractf(v);
-                const float4 q = *reinterpret_cast<const float4*>(base + __mul24(iv, rowb));
-                const float c0 = fmaf(fv, q.z, q.x);
-                const float c1 = fmaf(fv, q.w, q.y);
-                acc[k] = fmaf(c1, fw, fmaf(c0, gw, acc[k]));
+            for (int k0 = 0; k0 < BZ; k0 += kBatch) {
+                float4 q[kBatch];
+                float fv[kBatch];
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) {
+                    const float v = fmaf(float(k0 + j), dv, v0);
+                    fv[j] = __builtin_amdgcn_fractf(v);
+                    q[j] = *reinterpret_cast<const float4*>(base + __mul24(int(v), rowb));
+                }
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) {
+                    const float c0 = fmaf(fv[j], q[j].z, q[j].x);
+                    const float c1 = fmaf(fv[j], q[j].w, q[j].y);
+                    acc[k0 + j] = fmaf(c1, fw, fmaf(c0, gw, acc[k0 + j]));
+                }
             }
         } else {
             const char* base = static_cast<const char*>(lds) + iu * 8;
             const int rowb = rowq * 8;
             const half2_t gf = { (_Float16)gw, (_Float16)fw };
 #pragma unroll
-            for (int k = 0; k < BZ; k += 2) {
-                const float va = fmaf(float(k), dv, v0);
-                const float vb = fmaf(float(k + 1), dv, v0);
-                const int ia = int(va), ib = int(vb);
-                const half2_t fp = __builtin_bit_cast(half2_t,
-                    __builtin_amdgcn_cvt_pkrtz(__builtin_amdgcn_fractf(va), __builtin_amdgcn_fractf(vb)));
-                const uint2 qa = *reinterpret_cast<const uint2*>(base + __mul24(ia, rowb));
-                const uint2 qb = *reinterpret_cast<const uint2*>(base + __mul24(ib, rowb));
-                const half2_t fa = { fp.x, fp.x };
-                const half2_t fb = { fp.y, fp.y };
-                const half2_t ca = __builtin_bit_cast(half2_t, qa.y) * fa + __builtin_bit_cast(half2_t, qa.x);
-                const half2_t cb = __builtin_bit_cast(half2_t, qb.y) * fb + __builtin_bit_cast(half2_t, qb.x);
-                acc[k] = __builtin_amdgcn_fdot2(ca, gf, acc[k], false);
-                acc[k + 1] = __builtin_amdgcn_fdot2(cb, gf, acc[k + 1], false);
+            for (int k0 = 0; k0 < BZ; k0 += kBatch) {
+                uint2 q[kBatch];
+                half2_t fp[kBatch / 2];
+#pragma unroll
+                for (int j = 0; j < kBatch; j += 2) {
+                    const float va = fmaf(float(k0 + j), dv, v0);
+                    const float vb = fmaf(float(k0 + j + 1), dv, v0);
+                    fp[j / 2] = __builtin_bit_cast(half2_t,
+                        __builtin_amdgcn_cvt_pkrtz(__builtin_amdgcn_fractf(va), __builtin_amdgcn_fractf(vb)));
+                    q[j] = *reinterpret_cast<const uint2*>(base + __mul24(int(va), rowb));
+                    q[j + 1] = *reinterpret_cast<const uint2*>(base + __mul24(int(vb), rowb));
+                }
+#pragma unroll
+                for (int j = 0; j < kBatch; j += 2) {
+                    const half2_t fa = { fp[j / 2].x, fp[j / 2].x };
+                    const half2_t fb = { fp[j / 2].y, fp[j / 2].y };
+                    const half2_t ca = __builtin_bit_cast(half2_t, q[j].y) * fa + __builtin_bit_cast(half2_t, q[j].x);
+                    const half2_t cb = __builtin_bit_cast(half2_t, q[j + 1].y) * fb +
+                        __builtin_bit_cast(half2_t, q[j + 1].x);
+                    acc[k0 + j] = __builtin_amdgcn_fdot2(ca, gf, acc[k0 + j], false);
+                    acc[k0 + j + 1] = __builtin_amdgcn_fdot2(cb, gf, acc[k0 + j + 1], false);
+                }
             }
         }
     } else {
@@ -261,21 +440,20 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
         float un = c.cu[3] + c.cu[0] * X + c.cu[1] * Y + c.cu[2] * Z0;
         float vn = c.cv[3] + c.cv[0] * X + c.cv[1] * Y + c.cv[2] * Z0;
         const float dd = c.cd[2] * vz, du = c.cu[2] * vz, dvn = c.cv[2] * vz;
-        const int rowe = rowq;
 #pragma unroll 4
         for (int k = 0; k < BZ; ++k) {
-            const float r = 1.f / den;
+            const float r = __builtin_amdgcn_rcpf(den);
             const float u = un * r - ul0;
             const float v = vn * r - vl0;
             const int iu = int(u), iv = int(v);
             const float fu = u - float(iu), fv = v - float(iv);
             float c0, c1;
             if constexpr (STORAGE == YK_F32) {
-                const float4 q = static_cast<const float4*>(lds)[iv * rowe + iu];
+                const float4 q = static_cast<const float4*>(lds)[iv * rowq + iu];
                 c0 = fmaf(fv, q.z, q.x);
                 c1 = fmaf(fv, q.w, q.y);
             } else {
-                const uint2 q = static_cast<const uint2*>(lds)[iv * rowe + iu];
+                const uint2 q = static_cast<const uint2*>(lds)[iv * rowq + iu];
                 const half2_t h = __builtin_bit_cast(half2_t, q.x);
                 const half2_t d = __builtin_bit_cast(half2_t, q.y);
                 c0 = fmaf(fv, float(d.x), float(h.x));
@@ -308,6 +486,12 @@ fdk_bp_kernel(const void* __restrict__ proj, const float* __restrict__ row_scale
     const float Z0 = vol.oz + z0 * vol.vz;
     const size_t block = ((size_t)bzi * gridDim.y + byi) * gridDim.x + bxi;
     const short4* my_tiles = tiles + block * tile_stride + tile_offset;
+    const size_t view_elems = (size_t)nu * nv;
+    const size_t elem_bytes = STORAGE == YK_F32 ? 4 : 2;
+    auto view_base = [&](int i) {
+        return static_cast<const void*>(static_cast<const char*>(proj) + view_elems * elem_bytes * i);
+    };
+    auto view_scales = [&](int i) { return row_scales ? row_scales + (size_t)i * nv : nullptr; };
 
     float g = 1.f, inv_g = 1.f;
     if constexpr (STORAGE != YK_F32) {
@@ -325,20 +509,33 @@ fdk_bp_kernel(const void* __restrict__ proj, const float* __restrict__ row_scale
         for (int k = 0; k < BZ; ++k) { sum[k] = 0.f; comp[k] = 0.f; }
     }
 
-    const size_t view_elems = (size_t)nu * nv;
-    for (int i = 0; i < count; ++i) {
-        const short4 t = my_tiles[i];
-        if (t.z > 0) {
-            __syncthreads();
-            const void* view_base;
-            if constexpr (STORAGE == YK_F32) view_base = static_cast<const float*>(proj) + view_elems * i;
-            else view_base = static_cast<const unsigned short*>(proj) + view_elems * i;
-            stage_tile<STORAGE>(lds, view_base, row_scales ? row_scales + (size_t)i * nv : nullptr,
-                nu, nv, t, inv_g);
-            __syncthreads();
-            if (active)
-                accumulate_view<STORAGE, BZ, GENERAL>(lds, coeffs[i], t, X, Y, Z0, vol.vz, 1.f, acc);
+    // stage view 0
+    Prefetch<STORAGE> pf;
+    short4 t_cur = my_tiles[0];
+    if (t_cur.z > 0) {
+        const StageMap m = stage_map(t_cur);
+        if (m.fast) {
+            prefetch_tile<STORAGE>(pf, m, view_base(0), view_scales(0), nu, nv, t_cur);
+            commit_tile<STORAGE>(lds, pf, m, t_cur, nu, nv, inv_g);
+        } else {
+            stage_direct<STORAGE>(lds, view_base(0), view_scales(0), nu, nv, t_cur, inv_g);
         }
+    }
+    __syncthreads();
+
+#pragma unroll 1
+    for (int i = 0; i < count; ++i) {
+        const short4 t_next = (i + 1 < count) ? my_tiles[i + 1] : make_short4(0, 0, 0, 0);
+        StageMap m_next;
+        m_next.fast = false;
+        m_next.rows = 0;
+        if (t_next.z > 0) {
+            m_next = stage_map(t_next);
+            if (m_next.fast)
+                prefetch_tile<STORAGE>(pf, m_next, view_base(i + 1), view_scales(i + 1), nu, nv, t_next);
+        }
+        if (t_cur.z > 0 && active)
+            accumulate_view<STORAGE, BZ, GENERAL>(lds, coeffs[i], t_cur, X, Y, Z0, vol.vz, acc);
         if constexpr (KAHAN) {
             if ((i + 1) % kKahanViews == 0 || i + 1 == count) {
 #pragma unroll
@@ -351,6 +548,13 @@ fdk_bp_kernel(const void* __restrict__ proj, const float* __restrict__ row_scale
                 }
             }
         }
+        if (t_next.z > 0) {
+            __syncthreads();
+            if (m_next.fast) commit_tile<STORAGE>(lds, pf, m_next, t_next, nu, nv, inv_g);
+            else stage_direct<STORAGE>(lds, view_base(i + 1), view_scales(i + 1), nu, nv, t_next, inv_g);
+            __syncthreads();
+        }
+        t_cur = t_next;
     }
     if (!active) return;
     const size_t plane = (size_t)vol.nx * vol.ny;

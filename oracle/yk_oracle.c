@@ -9,6 +9,18 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int yko_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -535,26 +547,61 @@ void yko_fft(double* re, double* im, int n, int inverse)
  * crop chain (.../FDK/kernels/YkFDKFilterKernels.cu:32-140,
  * src/Filter/YkConvKernels.cu:109-220).  start_u is per view.
  */
+/* radix-2 FFT with a precomputed twiddle table c[k] + i s[k] = exp(-2 pi i k / N) */
+static void fft_table(double* re, double* im, int n, const double* c, const double* s, int inverse)
+{
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double t = re[i]; re[i] = re[j]; re[j] = t;
+            t = im[i]; im[i] = im[j]; im[j] = t;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        const int step = n / len;
+        for (int i = 0; i < n; i += len) {
+            for (int j = 0; j < len / 2; ++j) {
+                const double wr = c[j * step], wi = inverse ? -s[j * step] : s[j * step];
+                const int a = i + j, b = i + j + len / 2;
+                const double xr = re[b] * wr - im[b] * wi;
+                const double xi = re[b] * wi + im[b] * wr;
+                re[b] = re[a] - xr; im[b] = im[a] - xi;
+                re[a] += xr; im[a] += xi;
+            }
+        }
+    }
+}
+
 void yko_filter_rows(const double* in, int n_views, int nv, int nu,
     int N, const double* w, const int* start_u, int method, double* out)
 {
     double* h = NULL;
+    double* tc = NULL;
+    double* ts = NULL;
     if (method == 0) {
         h = (double*)malloc(sizeof(double) * (size_t)N);
         yko_filter_kernel_from_weights(w, N, h);
+    } else {
+        tc = (double*)malloc(sizeof(double) * (size_t)N);
+        ts = (double*)malloc(sizeof(double) * (size_t)N);
+        for (int k = 0; k < N; ++k) {
+            tc[k] = cos(-2.0 * M_PI * k / N);
+            ts[k] = sin(-2.0 * M_PI * k / N);
+        }
     }
     const long long rows = (long long)n_views * nv;
 #pragma omp parallel
     {
         double* re = (double*)malloc(sizeof(double) * (size_t)N);
         double* im = (double*)malloc(sizeof(double) * (size_t)N);
+        if (method == 0) {
 #pragma omp for schedule(dynamic, 16)
-        for (long long r = 0; r < rows; ++r) {
-            const int view = (int)(r / nv);
-            const int su = start_u[view];
-            const double* x = in + (size_t)r * nu;
-            double* y = out + (size_t)r * nu;
-            if (method == 0) {
+            for (long long r = 0; r < rows; ++r) {
+                const int su = start_u[r / nv];
+                const double* x = in + (size_t)r * nu;
+                double* y = out + (size_t)r * nu;
                 for (int a = 0; a < nu; ++a) {
                     const int j = su + a;
                     if (j < 0 || j >= N) { y[a] = 0.0; continue; }
@@ -568,21 +615,37 @@ void yko_filter_rows(const double* in, int n_views, int nv, int nu,
                     }
                     y[a] = acc;
                 }
-            } else {
+            }
+        } else {
+            /* two real rows per complex transform: the response is real and
+             * even, so Re and Im filter independently */
+#pragma omp for schedule(dynamic, 8)
+            for (long long p = 0; p < (rows + 1) / 2; ++p) {
+                const long long ra = 2 * p, rb = 2 * p + 1;
+                const int sa = start_u[ra / nv];
+                const int sb = rb < rows ? start_u[rb / nv] : 0;
                 for (int k = 0; k < N; ++k) { re[k] = 0.0; im[k] = 0.0; }
                 for (int b = 0; b < nu; ++b) {
-                    const int m = su + b;
-                    if (m >= 0 && m < N) re[m] = x[b];
+                    const int ma = sa + b;
+                    if (ma >= 0 && ma < N) re[ma] = in[(size_t)ra * nu + b];
+                    if (rb < rows) {
+                        const int mb = sb + b;
+                        if (mb >= 0 && mb < N) im[mb] = in[(size_t)rb * nu + b];
+                    }
                 }
-                yko_fft(re, im, N, 0);
+                fft_table(re, im, N, tc, ts, 0);
                 for (int k = 0; k < N; ++k) {
                     const double wk = w[k <= N / 2 ? k : N - k];
                     re[k] *= wk; im[k] *= wk;
                 }
-                yko_fft(re, im, N, 1);
+                fft_table(re, im, N, tc, ts, 1);
                 for (int a = 0; a < nu; ++a) {
-                    const int j = su + a;
-                    y[a] = (j < 0 || j >= N) ? 0.0 : re[j];
+                    const int ja = sa + a;
+                    out[(size_t)ra * nu + a] = (ja < 0 || ja >= N) ? 0.0 : re[ja];
+                    if (rb < rows) {
+                        const int jb = sb + a;
+                        out[(size_t)rb * nu + a] = (jb < 0 || jb >= N) ? 0.0 : im[jb];
+                    }
                 }
             }
         }
@@ -590,6 +653,8 @@ void yko_filter_rows(const double* in, int n_views, int nv, int nu,
         free(im);
     }
     free(h);
+    free(tc);
+    free(ts);
 }
 
 /* ------------------------------------------------------------------------ */

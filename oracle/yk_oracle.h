@@ -57,6 +57,9 @@ typedef struct {
     double cx, cy, cz;
 } yko_volume;
 
+/* OpenMP threads the oracle uses */
+int yko_num_threads(void);
+
 /* ---- geometry ---------------------------------------------------------- */
 int yko_build_circular_flat_views(
     int total_views, int views_per_turn, double start_angle_rad, int direction,

@@ -86,6 +86,7 @@ def lib():
             "yko_phantom_ellipsoids": (None, [dp, c_int, vp, c_int, fp]),
             "yko_ellipsoid_line_integrals": (None, [dp, c_int, dp, c_int, c_int, c_int, dp]),
             "yko_tv_value_grad": (None, [dp, vp, c_double, dp, dp]),
+            "yko_num_threads": (c_int, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -287,6 +288,10 @@ def ellipsoid_line_integrals(ellipsoids, views, nu, nv) -> np.ndarray:
     out = np.zeros((views.shape[0], nv, nu))
     lib().yko_ellipsoid_line_integrals(_dp(e), e.shape[0], _dp(views), views.shape[0], nu, nv, _dp(out))
     return out
+
+
+def num_threads() -> int:
+    return lib().yko_num_threads()
 
 
 def tv_value_grad(x, vol: Volume, eps):
