@@ -239,10 +239,13 @@ void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_sca
     const size_t quad = storage == YK_F32 ? 16 : 8;
     if (size_t(lds_quads) * quad > 160 * 1024) fail(YK_ERR_UNSUPPORTED, "footprint exceeds LDS");
     hipStream_t s = ctx->stream;
-    if (storage != YK_F32) launch_max_scale(d_scales, count * nv, d_maxes.get() + 4, s);
+    if (storage != YK_F32) {
+        d_rowfac.resize(std::max(d_rowfac.size(), size_t(count) * nv));
+        launch_row_factors(d_scales, count * nv, storage, d_maxes.get() + 4, d_rowfac.get(), s);
+    }
     BpLaunch p{};
     p.proj = d_filtered;
-    p.row_scales = storage == YK_F32 ? nullptr : d_scales;
+    p.row_factors = storage == YK_F32 ? nullptr : d_rowfac.get();
     p.gscale_bits = d_maxes.get() + 4;
     p.coeffs = d_coeffs.get() + first;
     p.tiles = d_tiles.get();

@@ -37,6 +37,7 @@ struct yk_fdk {
     yk::DeviceArray<float> d_parker;
     yk::DeviceArray<short4> d_tiles;
     yk::DeviceArray<int> d_maxes;      // [0..3] footprint maxima, [4] global row-scale bits
+    yk::DeviceArray<unsigned> d_rowfac; // normalized row factors of the views being backprojected
     int nbx = 0, nby = 0, nbz = 0, bz = 0;
     int lds_quads = 0;
 
@@ -71,6 +72,6 @@ struct yk_projector {
     yk::DeviceArray<double> d_c64;
     yk::DeviceArray<float> d_vol_t;
     yk::DeviceArray<double> d_partials;
-    yk_fdk* fdk_op = nullptr;          // FDK operator-form backprojector, built on demand
-    ~yk_projector();
+    yk::DeviceArray<yk::OpBpViewDev> d_opviews;  // operator-form FDK backprojection records
+    std::vector<double> op_scale;      // per first view: vx vy vz / (du dv)
 };

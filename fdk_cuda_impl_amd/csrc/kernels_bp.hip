@@ -5,9 +5,11 @@
 // BZ accumulators in registers for the whole view loop, so the volume is read
 // and written once per launch.  Per view the workgroup stages the detector
 // footprint of its voxel block into LDS as "quads": for texel (r, c) the four
-// floats (q[r][c], q[r][c+1], q[r+1][c]-q[r][c], q[r+1][c+1]-q[r][c+1]), so
-// one 16-byte LDS read feeds a whole bilinear sample.  The footprint of every
+// values (q[r][c], q[r][c+1], q[r+1][c]-q[r][c], q[r+1][c+1]-q[r][c+1]), so
+// one LDS read feeds a whole bilinear sample.  The footprint of every
 // (block, view) pair is computed once per plan in FP64 (bp_tiles_kernel).
+// The texels of view i+1 are loaded into registers while view i is
+// accumulated, then written to LDS between two barriers.
 //
 // For a detector whose rows stay parallel to the rotation axis (the regular
 // circular geometry; Cu_z = Cd_z = 0 exactly) u, the u-weights and the
@@ -41,8 +43,37 @@ constexpr int kBX = 16;
 constexpr int kBY = 16;
 constexpr int kThreads = kBX * kBY;
 constexpr int kKahanViews = 16;
+constexpr int kMaxRows = 16;  // texel rows a thread prefetches (quad rows + 1)
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// staging layout of a tile of qw x qh quads: lanes take `span` consecutive
+// columns, `chunks` row ranges of `rows_per` quad rows each
+struct StageShape {
+    int span, chunks, rows_per;
+};
+
+__host__ __device__ __forceinline__ StageShape stage_shape(int qw, int qh)
+{
+    StageShape s;
+    s.span = qw < kThreads ? qw : kThreads;
+    s.chunks = kThreads / s.span;
+    s.rows_per = (qh + s.chunks - 1) / s.chunks;
+    return s;
+}
+
+// LDS row pitch in quads: odd, so that lanes reading the same column of
+// neighbouring rows fall on different banks
+__host__ __device__ __forceinline__ int lds_pitch(int qw) { return qw | 1; }
+
+// quads of LDS a tile needs: the staging writes whole row ranges, so the
+// last range may spill past qh rows
+__host__ __device__ __forceinline__ int stage_quads(int qw, int qh)
+{
+    const StageShape s = stage_shape(qw, qh);
+    const bool fast = qw <= kThreads && s.rows_per + 1 <= kMaxRows;
+    return lds_pitch(qw) * (fast ? s.chunks * s.rows_per : qh);
+}
 
 // ---------------------------------------------------------------------------
 // footprint table: per (block, view) the quad range [u0, u0+w) x [v0, v0+h)
@@ -96,7 +127,7 @@ __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, Vol
                 t.w = short(fv1 - fv0 + 1.0);
                 atomicMax(&maxes[0], int(t.z));
                 atomicMax(&maxes[1], int(t.w));
-                atomicMax(&maxes[2], int(t.z) * int(t.w));
+                atomicMax(&maxes[2], stage_quads(t.z, t.w));
             }
         }
     }
@@ -121,46 +152,68 @@ __global__ void max_scale_kernel(const float* __restrict__ s, int n, int* __rest
     if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_int(m));  // m >= 0
 }
 
+// normalized row factors: scale / (global max scale), a power of two <= 1.
+// half storage: packed (f, f) as two halves (rows below 2^-24 of the largest
+// row scale become 0; their texels lie below 2^-22 of the largest texel);
+// bfloat16 storage: the float factor.
+__global__ void row_factor_kernel(const float* __restrict__ s, int n, const int* __restrict__ gbits,
+    int storage, unsigned* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float g = __int_as_float(*gbits);
+    if (!(g > 0.f)) g = 1.f;
+    const float f = s[i] / g;
+    if (storage == YK_F16) {
+        const _Float16 h = (_Float16)f;
+        const half2_t p = { h, h };
+        out[i] = __builtin_bit_cast(unsigned, p);
+    } else {
+        out[i] = __float_as_uint(f);
+    }
+}
+
 // ---------------------------------------------------------------------------
-// staging: the quads of tile (u0, v0, qw, qh) of one view.  Lanes take
-// consecutive quad columns; each thread owns a run of quad rows in one column
-// and slides a two-row window down it.  For tiles up to kMaxRows rows per
-// thread the texels of the NEXT view are fetched into registers before the
-// current view is accumulated, so global latency hides behind the LDS work.
+// staging
 // ---------------------------------------------------------------------------
-constexpr int kMaxRows = 16;  // texel rows a thread can hold (quad rows + 1)
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, 32-bit data format
+
+// Buffer resource over `bytes` bytes: a load at an offset at or past the end
+// (negative offsets wrap to large unsigned values) returns 0, which zeroes
+// every texel of a row above or below the detector without a branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(bytes), kRsrcWord3);
+}
 
 struct StageMap {
-    int col;      // quad column
-    int r0;       // first quad row
-    int rows;     // quad rows written by this thread (0: idle)
-    int rows_per; // quad rows per thread (uniform)
-    bool fast;    // fits the register prefetch
+    StageShape s;
+    int col;      // quad column of this lane within the tile
+    int cl;       // first loaded texel column, clamped into [0, nu - 2]
+    int shift;    // requested - loaded first column: -1, 0 or 1 on the detector edges
+    int r0;       // first quad row of this lane
+    bool lane_on; // lane has a row range
+    bool fast;    // tile fits the register prefetch
 };
 
-__device__ __forceinline__ StageMap stage_map(short4 t)
+__device__ __forceinline__ StageMap stage_map(short4 t, int nu)
 {
     StageMap m;
-    const int qw = t.z, qh = t.w;
-    const int span = min(qw, kThreads);
-    const int chunks = kThreads / span;
-    const int rows_per = (qh + chunks - 1) / chunks;
-    const int rc = int(threadIdx.x) / span;
-    m.col = int(threadIdx.x) % span;
-    m.r0 = rc * rows_per;
-    m.rows = rc < chunks ? max(0, min(rows_per, qh - m.r0)) : 0;
-    m.rows_per = rows_per;
-    m.fast = qw <= kThreads && rows_per + 1 <= kMaxRows;
+    m.s = stage_shape(t.z, t.w);
+    // threadIdx.x / span by a 16-bit reciprocal: exact for threadIdx.x < 256
+    // and span <= 256 (the rounding excess stays below 1/256)
+    const unsigned magic = (65536u + unsigned(m.s.span) - 1u) / unsigned(m.s.span);
+    const int rc = int((threadIdx.x * magic) >> 16);
+    m.col = int(threadIdx.x) - rc * m.s.span;
+    const int col = t.x + m.col;
+    m.cl = min(max(col, 0), nu - 2);
+    m.shift = col - m.cl;
+    m.r0 = rc * m.s.rows_per;
+    m.lane_on = rc < m.s.chunks;
+    m.fast = t.z <= kThreads && m.s.rows_per + 1 <= kMaxRows;
     return m;
 }
 
-__device__ __forceinline__ bool tile_inside(short4 t, int nu, int nv)
-{
-    return t.x >= 0 && t.y >= 0 && t.x + t.z + 1 <= nu && t.y + t.w + 1 <= nv;
-}
-
-// raw texel pair (col, col+1) of one detector row: float2 for float storage,
-// two 16-bit patterns packed lo/hi for half / bfloat16; zero outside
 template <int STORAGE>
 struct Pair;
 template <>
@@ -170,193 +223,152 @@ struct Pair<YK_F16> { using T = unsigned; };
 template <>
 struct Pair<YK_BF16> { using T = unsigned; };
 
+// texel pair (col, col + 1) of one row at byte offset `off`: float2, or the
+// two 16-bit texels as one dword (lo = col), read by a single dword load at a
+// 2-byte aligned offset (the queues of ROCm run with unaligned buffer access
+// enabled).  The pair never straddles two rows (the column is clamped), so it
+// is either wholly in range or wholly zero.
 template <int STORAGE>
-__device__ __forceinline__ typename Pair<STORAGE>::T fetch_pair(const void* __restrict__ base, int nu,
-    int nv, int row, int col, bool inside)
+__device__ __forceinline__ typename Pair<STORAGE>::T load_pair(__amdgpu_buffer_rsrc_t r, unsigned off)
 {
     if constexpr (STORAGE == YK_F32) {
-        const float* p = static_cast<const float*>(base);
-        if (inside) {
-            const float* r = p + (size_t)row * nu + col;
-            return make_float2(r[0], r[1]);
-        }
-        float2 v = make_float2(0.f, 0.f);
-        if (row >= 0 && row < nv) {
-            const float* r = p + (size_t)row * nu;
-            if (col >= 0 && col < nu) v.x = r[col];
-            if (col + 1 >= 0 && col + 1 < nu) v.y = r[col + 1];
-        }
-        return v;
+        return make_float2(__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0)),
+                           __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off + 4, 0, 0)));
     } else {
-        const unsigned short* p = static_cast<const unsigned short*>(base);
-        unsigned lo = 0, hi = 0;
-        if (inside) {
-            const unsigned short* r = p + (size_t)row * nu + col;
-            lo = r[0];
-            hi = r[1];
-        } else if (row >= 0 && row < nv) {
-            const unsigned short* r = p + (size_t)row * nu;
-            if (col >= 0 && col < nu) lo = r[col];
-            if (col + 1 >= 0 && col + 1 < nu) hi = r[col + 1];
-        }
-        return lo | (hi << 16);
+        return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
     }
 }
 
-// row normalizer: stored * scale / global scale (an exact power of two)
-__device__ __forceinline__ float row_factor(const float* __restrict__ scales, int nv, int row, float inv_g)
+// byte offset of a row factor; the empty asm keeps the compiler from moving
+// the row step into the instruction offset, so a negative row stays a
+// wrapped (out-of-range) offset rather than summing back into range
+__device__ __forceinline__ unsigned factor_offset(int row)
 {
-    return (row >= 0 && row < nv) ? scales[row] * inv_g : 0.f;
+    unsigned o = unsigned(row) * 4u;
+    asm volatile("" : "+v"(o));
+    return o;
 }
 
-// normalized half pair of a raw pair
-template <int STORAGE>
-__device__ __forceinline__ half2_t to_half2(unsigned raw, float f)
-{
-    if constexpr (STORAGE == YK_F16) {
-        const half2_t h = __builtin_bit_cast(half2_t, raw);
-        const _Float16 fh = (_Float16)f;  // power of two: exact when representable
-        const half2_t ff = { fh, fh };
-        return h * ff;
-    } else {
-        const float a = __uint_as_float(raw << 16) * f;
-        const float b = __uint_as_float(raw & 0xffff0000u) * f;
-        const half2_t r = { (_Float16)a, (_Float16)b };
-        return r;
-    }
-}
-
-template <int STORAGE>
-__device__ __forceinline__ void write_quad(void* lds, int index, const typename Pair<STORAGE>::T& top,
-    const typename Pair<STORAGE>::T& bot, float ftop, float fbot)
-{
-    if constexpr (STORAGE == YK_F32) {
-        static_cast<float4*>(lds)[index] = make_float4(top.x, top.y, bot.x - top.x, bot.y - top.y);
-    } else {
-        const half2_t a = to_half2<STORAGE>(top, ftop);
-        const half2_t b = to_half2<STORAGE>(bot, fbot);
-        const half2_t d = b - a;
-        uint2 q;
-        q.x = __builtin_bit_cast(unsigned, a);
-        q.y = __builtin_bit_cast(unsigned, d);
-        static_cast<uint2*>(lds)[index] = q;
-    }
-}
-
-// direct staging (no register prefetch), used for very tall or wide tiles
-template <int STORAGE>
-__device__ void stage_direct(void* lds, const void* __restrict__ base, const float* __restrict__ scales,
-    int nu, int nv, short4 t, float inv_g)
-{
-    const int qw = t.z, qh = t.w;
-    const bool inside = tile_inside(t, nu, nv);
-    const int span = min(qw, kThreads);
-    const int chunks = kThreads / span;
-    const int rows_per = (qh + chunks - 1) / chunks;
-    const int rc = int(threadIdx.x) / span;
-    if (rc >= chunks) return;
-    const int r0 = rc * rows_per;
-    const int r1 = min(r0 + rows_per, qh);
-    for (int c = int(threadIdx.x) % span; c < qw; c += span) {
-        if (r0 >= r1) break;
-        auto top = fetch_pair<STORAGE>(base, nu, nv, t.y + r0, t.x + c, inside);
-        float ft = 1.f;
-        if constexpr (STORAGE != YK_F32) ft = row_factor(scales, nv, t.y + r0, inv_g);
-        for (int r = r0; r < r1; ++r) {
-            auto bot = fetch_pair<STORAGE>(base, nu, nv, t.y + r + 1, t.x + c, inside);
-            float fb = 1.f;
-            if constexpr (STORAGE != YK_F32) fb = row_factor(scales, nv, t.y + r + 1, inv_g);
-            write_quad<STORAGE>(lds, r * qw + c, top, bot, ft, fb);
-            top = bot;
-            ft = fb;
-        }
-    }
-}
-
-// Register prefetch: loads only, no use of the loaded values, so the compiler
-// can leave every load in flight until commit.  Rows and columns are clamped
-// into the detector so each load is in bounds; commit zeroes what lies
-// outside.  Requires nu >= 2.
 template <int STORAGE>
 struct Prefetch {
     typename Pair<STORAGE>::T v[kMaxRows];
-    float f[STORAGE == YK_F32 ? 1 : kMaxRows];
+    unsigned f[STORAGE == YK_F32 ? 1 : kMaxRows];
 };
 
-template <int STORAGE>
-__device__ __forceinline__ typename Pair<STORAGE>::T raw_pair(const void* __restrict__ base, int nu,
-    int row, int col)
-{
-    if constexpr (STORAGE == YK_F32) {
-        const float* r = static_cast<const float*>(base) + (size_t)row * nu + col;
-        return make_float2(r[0], r[1]);
-    } else {
-        // two adjacent 16-bit texels as one (possibly 2-byte aligned) dword
-        const unsigned short* r = static_cast<const unsigned short*>(base) + (size_t)row * nu + col;
-        return unsigned(r[0]) | (unsigned(r[1]) << 16);
-    }
-}
-
+// Loads only: nothing consumes the loaded values before commit, so every load
+// stays in flight while the current view is accumulated.
 template <int STORAGE>
 __device__ __forceinline__ void prefetch_tile(Prefetch<STORAGE>& pf, const StageMap& m,
-    const void* __restrict__ base, const float* __restrict__ scales, int nu, int nv, short4 t)
+    __amdgpu_buffer_rsrc_t data, __amdgpu_buffer_rsrc_t fac, int nu, short4 t)
 {
-    const int cl = min(max(t.x + m.col, 0), nu - 2);
-    const int rbase = t.y + m.r0;
-    const int rows_per = m.rows_per;  // uniform across the workgroup
+    if (!m.lane_on) return;
+    constexpr int eb = STORAGE == YK_F32 ? 4 : 2;
+    const int row0 = t.y + m.r0;
+    const int off0 = (row0 * nu + m.cl) * eb;
+    const int rowb = nu * eb;
+    const int rows_per = m.s.rows_per;  // uniform across the workgroup
 #pragma unroll
     for (int j = 0; j < kMaxRows; ++j) {
         if (j <= rows_per) {
-            const int row = min(max(rbase + j, 0), nv - 1);
-            pf.v[j] = raw_pair<STORAGE>(base, nu, row, cl);
-            if constexpr (STORAGE != YK_F32) pf.f[j] = scales[row];
+            pf.v[j] = load_pair<STORAGE>(data, unsigned(off0 + j * rowb));
+            if constexpr (STORAGE != YK_F32)
+                pf.f[j] = __builtin_amdgcn_raw_buffer_load_b32(fac, factor_offset(row0 + j), 0, 0);
         }
     }
 }
 
-// shift = requested first column - loaded first column (0 inside the detector)
+// move the pair onto the requested columns, zeroing the off-detector texel
 template <int STORAGE>
-__device__ __forceinline__ typename Pair<STORAGE>::T fix_pair(typename Pair<STORAGE>::T v, int shift, bool row_ok)
+__device__ __forceinline__ typename Pair<STORAGE>::T shift_pair(typename Pair<STORAGE>::T v, int shift)
 {
     if constexpr (STORAGE == YK_F32) {
         float2 r;
         r.x = shift == 0 ? v.x : (shift == 1 ? v.y : 0.f);
         r.y = shift == 0 ? v.y : (shift == -1 ? v.x : 0.f);
-        return row_ok ? r : make_float2(0.f, 0.f);
+        return r;
     } else {
-        unsigned r = shift == 0 ? v : (shift == 1 ? (v >> 16) : (shift == -1 ? (v << 16) : 0u));
-        return row_ok ? r : 0u;
+        return shift == 0 ? v : (shift == 1 ? (v >> 16) : (shift == -1 ? (v << 16) : 0u));
     }
 }
 
+// normalized half pair of a raw pair and its row factor
 template <int STORAGE>
-__device__ __forceinline__ void commit_tile(void* lds, const Prefetch<STORAGE>& pf, const StageMap& m,
-    short4 t, int nu, int nv, float inv_g)
+__device__ __forceinline__ half2_t to_half2(unsigned raw, unsigned f)
 {
-    if (m.rows <= 0) return;
-    const int col0 = t.x + m.col;
-    const int shift = col0 - min(max(col0, 0), nu - 2);
-    const int rbase = t.y + m.r0;
-    if (tile_inside(t, nu, nv)) {
+    if constexpr (STORAGE == YK_F16) {
+        return __builtin_bit_cast(half2_t, raw) * __builtin_bit_cast(half2_t, f);
+    } else {
+        const float ff = __uint_as_float(f);
+        return __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(
+            __uint_as_float(raw << 16) * ff, __uint_as_float(raw & 0xffff0000u) * ff));
+    }
+}
+
+// Write quad rows [r0, r0 + rows_per) of this lane's column.  Rows past the
+// tile land in the LDS padding (stage_quads) and are never read.
+template <int STORAGE>
+__device__ __forceinline__ void commit_tile(void* lds, Prefetch<STORAGE>& pf, const StageMap& m, short4 t)
+{
+    if (!m.lane_on) return;
+    const int rows_per = m.s.rows_per;
+    if (m.shift != 0) {  // lanes on the detector's first or last column only
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j)
+            if (j <= rows_per) pf.v[j] = shift_pair<STORAGE>(pf.v[j], m.shift);
+    }
+    constexpr int qb = STORAGE == YK_F32 ? 16 : 8;
+    const int pitch = lds_pitch(t.z);
+    char* dst = static_cast<char*>(lds) + (m.r0 * pitch + m.col) * qb;
+    const int rowb = pitch * qb;
+    if constexpr (STORAGE == YK_F32) {
 #pragma unroll
         for (int j = 0; j + 1 < kMaxRows; ++j) {
-            if (j < m.rows) {
-                float ft = 1.f, fb = 1.f;
-                if constexpr (STORAGE != YK_F32) { ft = pf.f[j] * inv_g; fb = pf.f[j + 1] * inv_g; }
-                write_quad<STORAGE>(lds, (m.r0 + j) * t.z + m.col, pf.v[j], pf.v[j + 1], ft, fb);
+            if (j < rows_per) {
+                const float2 a = pf.v[j], b = pf.v[j + 1];
+                *reinterpret_cast<float4*>(dst + j * rowb) = make_float4(a.x, a.y, b.x - a.x, b.y - a.y);
             }
         }
     } else {
+        half2_t x[kMaxRows];
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j)
+            if (j <= rows_per) x[j] = to_half2<STORAGE>(pf.v[j], pf.f[j]);
 #pragma unroll
         for (int j = 0; j + 1 < kMaxRows; ++j) {
-            if (j < m.rows) {
-                const int rt = rbase + j, rb = rt + 1;
-                const bool okt = rt >= 0 && rt < nv, okb = rb >= 0 && rb < nv;
-                float ft = 1.f, fb = 1.f;
-                if constexpr (STORAGE != YK_F32) { ft = pf.f[j] * inv_g; fb = pf.f[j + 1] * inv_g; }
-                write_quad<STORAGE>(lds, (m.r0 + j) * t.z + m.col, fix_pair<STORAGE>(pf.v[j], shift, okt),
-                    fix_pair<STORAGE>(pf.v[j + 1], shift, okb), ft, fb);
+            if (j < rows_per) {
+                uint2 q;
+                q.x = __builtin_bit_cast(unsigned, x[j]);
+                q.y = __builtin_bit_cast(unsigned, x[j + 1] - x[j]);
+                *reinterpret_cast<uint2*>(dst + j * rowb) = q;
             }
+        }
+    }
+}
+
+// direct staging for tiles too tall or wide for the register prefetch
+template <int STORAGE>
+__device__ void stage_direct(void* lds, __amdgpu_buffer_rsrc_t data, __amdgpu_buffer_rsrc_t fac,
+    int nu, short4 t)
+{
+    constexpr int eb = STORAGE == YK_F32 ? 4 : 2;
+    const int qw = t.z, qh = t.w, pitch = lds_pitch(qw);
+    for (int e = threadIdx.x; e < qw * qh; e += kThreads) {
+        const int r = e / qw, c = e - r * qw;
+        const int slot = r * pitch + c;
+        const int col = t.x + c;
+        const int cl = min(max(col, 0), nu - 2);
+        const int rt = t.y + r;
+        const auto a = shift_pair<STORAGE>(load_pair<STORAGE>(data, unsigned((rt * nu + cl) * eb)), col - cl);
+        const auto b = shift_pair<STORAGE>(load_pair<STORAGE>(data, unsigned(((rt + 1) * nu + cl) * eb)), col - cl);
+        if constexpr (STORAGE == YK_F32) {
+            static_cast<float4*>(lds)[slot] = make_float4(a.x, a.y, b.x - a.x, b.y - a.y);
+        } else {
+            const half2_t xa = to_half2<STORAGE>(a, __builtin_amdgcn_raw_buffer_load_b32(fac, factor_offset(rt), 0, 0));
+            const half2_t xb = to_half2<STORAGE>(b, __builtin_amdgcn_raw_buffer_load_b32(fac, factor_offset(rt + 1), 0, 0));
+            uint2 q;
+            q.x = __builtin_bit_cast(unsigned, xa);
+            q.y = __builtin_bit_cast(unsigned, xb - xa);
+            static_cast<uint2*>(lds)[slot] = q;
         }
     }
 }
@@ -371,7 +383,7 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
     float X, float Y, float Z0, float vz, float (&acc)[BZ])
 {
     const float ul0 = float(t.x), vl0 = float(t.y);
-    const int rowq = t.z;
+    const int rowq = lds_pitch(t.z);
     // distance weight: the radial ray has no z component, so it is z-invariant
     const float depth = c.cr[3] + c.cr[0] * X + c.cr[1] * Y;
     const float rd = __builtin_amdgcn_rcpf(depth);
@@ -469,7 +481,7 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
 
 template <int STORAGE, int BZ, bool GENERAL, bool KAHAN>
 __global__ void __launch_bounds__(kThreads)
-fdk_bp_kernel(const void* __restrict__ proj, const float* __restrict__ row_scales,
+fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_factors,
     const int* __restrict__ gscale_bits, const BpCoeff* __restrict__ coeffs,
     const short4* __restrict__ tiles, int tile_stride, int tile_offset,
     float* __restrict__ volume, VolumeGeom vol, int nu, int nv, int count, int accumulate)
@@ -486,39 +498,45 @@ fdk_bp_kernel(const void* __restrict__ proj, const float* __restrict__ row_scale
     const float Z0 = vol.oz + z0 * vol.vz;
     const size_t block = ((size_t)bzi * gridDim.y + byi) * gridDim.x + bxi;
     const short4* my_tiles = tiles + block * tile_stride + tile_offset;
-    const size_t view_elems = (size_t)nu * nv;
-    const size_t elem_bytes = STORAGE == YK_F32 ? 4 : 2;
-    auto view_base = [&](int i) {
-        return static_cast<const void*>(static_cast<const char*>(proj) + view_elems * elem_bytes * i);
+    const unsigned view_bytes = unsigned(nu) * unsigned(nv) * (STORAGE == YK_F32 ? 4u : 2u);
+    auto data_rsrc = [&](int i) {
+        return make_rsrc(static_cast<const char*>(proj) + (size_t)view_bytes * i, view_bytes);
     };
-    auto view_scales = [&](int i) { return row_scales ? row_scales + (size_t)i * nv : nullptr; };
+    auto fac_rsrc = [&](int i) {
+        return make_rsrc(row_factors + (STORAGE == YK_F32 ? 0 : (size_t)nv * i), unsigned(nv) * 4u);
+    };
 
-    float g = 1.f, inv_g = 1.f;
+    float g = 1.f;
     if constexpr (STORAGE != YK_F32) {
         g = __int_as_float(*gscale_bits);
         if (!(g > 0.f)) g = 1.f;
-        inv_g = 1.f / g;
     }
 
     float acc[BZ];
 #pragma unroll
     for (int k = 0; k < BZ; ++k) acc[k] = 0.f;
-    float sum[KAHAN ? BZ : 1], comp[KAHAN ? BZ : 1];
+    // Kahan state: FP32 sums, compensations truncated to bfloat16 and packed
+    // two per register (a compensation is the rounding error of its sum, so
+    // 8 bits of it leave the compensated sum within ~2^-8 ulp)
+    float sum[KAHAN ? BZ : 1];
+    unsigned comp[KAHAN ? BZ / 2 : 1];
     if constexpr (KAHAN) {
 #pragma unroll
-        for (int k = 0; k < BZ; ++k) { sum[k] = 0.f; comp[k] = 0.f; }
+        for (int k = 0; k < BZ; ++k) sum[k] = 0.f;
+#pragma unroll
+        for (int k = 0; k < BZ / 2; ++k) comp[k] = 0u;
     }
 
     // stage view 0
     Prefetch<STORAGE> pf;
     short4 t_cur = my_tiles[0];
     if (t_cur.z > 0) {
-        const StageMap m = stage_map(t_cur);
+        const StageMap m = stage_map(t_cur, nu);
         if (m.fast) {
-            prefetch_tile<STORAGE>(pf, m, view_base(0), view_scales(0), nu, nv, t_cur);
-            commit_tile<STORAGE>(lds, pf, m, t_cur, nu, nv, inv_g);
+            prefetch_tile<STORAGE>(pf, m, data_rsrc(0), fac_rsrc(0), nu, t_cur);
+            commit_tile<STORAGE>(lds, pf, m, t_cur);
         } else {
-            stage_direct<STORAGE>(lds, view_base(0), view_scales(0), nu, nv, t_cur, inv_g);
+            stage_direct<STORAGE>(lds, data_rsrc(0), fac_rsrc(0), nu, t_cur);
         }
     }
     __syncthreads();
@@ -526,32 +544,35 @@ fdk_bp_kernel(const void* __restrict__ proj, const float* __restrict__ row_scale
 #pragma unroll 1
     for (int i = 0; i < count; ++i) {
         const short4 t_next = (i + 1 < count) ? my_tiles[i + 1] : make_short4(0, 0, 0, 0);
-        StageMap m_next;
-        m_next.fast = false;
-        m_next.rows = 0;
+        StageMap m_next{};
         if (t_next.z > 0) {
-            m_next = stage_map(t_next);
+            m_next = stage_map(t_next, nu);
             if (m_next.fast)
-                prefetch_tile<STORAGE>(pf, m_next, view_base(i + 1), view_scales(i + 1), nu, nv, t_next);
+                prefetch_tile<STORAGE>(pf, m_next, data_rsrc(i + 1), fac_rsrc(i + 1), nu, t_next);
         }
         if (t_cur.z > 0 && active)
             accumulate_view<STORAGE, BZ, GENERAL>(lds, coeffs[i], t_cur, X, Y, Z0, vol.vz, acc);
         if constexpr (KAHAN) {
             if ((i + 1) % kKahanViews == 0 || i + 1 == count) {
 #pragma unroll
-                for (int k = 0; k < BZ; ++k) {
-                    const float yk = acc[k] - comp[k];
-                    const float tk = sum[k] + yk;
-                    comp[k] = (tk - sum[k]) - yk;
-                    sum[k] = tk;
+                for (int k = 0; k < BZ; k += 2) {
+                    const float c0 = __uint_as_float(comp[k / 2] << 16);
+                    const float c1 = __uint_as_float(comp[k / 2] & 0xffff0000u);
+                    const float y0 = acc[k] - c0, y1 = acc[k + 1] - c1;
+                    const float t0 = sum[k] + y0, t1 = sum[k + 1] + y1;
+                    const float n0 = (t0 - sum[k]) - y0, n1 = (t1 - sum[k + 1]) - y1;
+                    comp[k / 2] = (__float_as_uint(n0) >> 16) | (__float_as_uint(n1) & 0xffff0000u);
+                    sum[k] = t0;
+                    sum[k + 1] = t1;
                     acc[k] = 0.f;
+                    acc[k + 1] = 0.f;
                 }
             }
         }
         if (t_next.z > 0) {
             __syncthreads();
-            if (m_next.fast) commit_tile<STORAGE>(lds, pf, m_next, t_next, nu, nv, inv_g);
-            else stage_direct<STORAGE>(lds, view_base(i + 1), view_scales(i + 1), nu, nv, t_next, inv_g);
+            if (m_next.fast) commit_tile<STORAGE>(lds, pf, m_next, t_next);
+            else stage_direct<STORAGE>(lds, data_rsrc(i + 1), fac_rsrc(i + 1), nu, t_next);
             __syncthreads();
         }
         t_cur = t_next;
@@ -564,7 +585,8 @@ fdk_bp_kernel(const void* __restrict__ proj, const float* __restrict__ row_scale
         const int z = z0 + k;
         if (z >= vol.nz) break;
         float val;
-        if constexpr (KAHAN) val = sum[k] - comp[k];
+        if constexpr (KAHAN)
+            val = sum[k] - __uint_as_float((k & 1) ? (comp[k / 2] & 0xffff0000u) : (comp[k / 2] << 16));
         else val = acc[k];
         val *= g;
         float* dst = volume + (size_t)z * plane + col;
@@ -580,7 +602,7 @@ void launch_bz(const BpLaunch& p, const int* gbits, hipStream_t s)
     auto k = fdk_bp_kernel<STORAGE, BZ, GENERAL, KAHAN>;
     YK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL(k, grid, block, lds, s, p.proj, p.row_scales, gbits, p.coeffs, p.tiles,
+    hipLaunchKernelGGL(k, grid, block, lds, s, p.proj, p.row_factors, gbits, p.coeffs, p.tiles,
         p.tile_view_stride, p.tile_view_offset, p.volume, p.vol, p.nu, p.nv, p.count, p.accumulate);
 }
 
@@ -621,17 +643,24 @@ void launch_bp_tiles(const TileLaunch& p, hipStream_t s)
     YK_HIP(hipGetLastError());
 }
 
-void launch_max_scale(const float* scales, int n, int* out_bits, hipStream_t s)
+void launch_row_factors(const float* scales, int n, int storage, int* gbits, unsigned* factors,
+    hipStream_t s)
 {
-    hipLaunchKernelGGL(set_ints_kernel, dim3(1), dim3(64), 0, s, out_bits, 1, 0);
+    hipLaunchKernelGGL(set_ints_kernel, dim3(1), dim3(64), 0, s, gbits, 1, 0);
     const int threads = 256;
     const int blocks = std::max(1, std::min(1024, (n + threads - 1) / threads));
-    hipLaunchKernelGGL(max_scale_kernel, dim3(blocks), dim3(threads), 0, s, scales, n, out_bits);
+    hipLaunchKernelGGL(max_scale_kernel, dim3(blocks), dim3(threads), 0, s, scales, n, gbits);
+    hipLaunchKernelGGL(row_factor_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, s, scales, n,
+        gbits, storage, factors);
     YK_HIP(hipGetLastError());
 }
 
 void launch_fdk_bp(const BpLaunch& p, hipStream_t s)
 {
+    // staging offsets are 32-bit byte offsets of rows in [-4400, nv + 4400)
+    if ((size_t(p.nv) + 8800) * size_t(p.nu) * 4 >= (size_t(1) << 31))
+        fail(YK_ERR_UNSUPPORTED, "detector too large for 32-bit staging offsets");
+    if (p.storage != YK_F32 && !p.row_factors) fail(YK_ERR_INVALID_BUFFER, "row factors missing");
     const int bz = p.bz;
     const int* gbits = p.gscale_bits;
     switch (p.storage) {

@@ -396,6 +396,79 @@ siddon_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
     *dst = accumulate ? *dst + acc : acc;
 }
 
+// Operator-form FDK backprojections, the iterative B of the reference's
+// automatic flat selection (fdk_bp_kernel_fdk_weight_v2 and
+// fdk_bp_kernel_matched_weight, src/FlatFpBp/BP/kernels/YkBpFdkKernel.cu:228-466):
+// unfiltered views, exact bilinear sample at the projected voxel centre (zero
+// outside the detector), per-sample weight
+//   FDK:     SOD^2 dtheta fscale / (Cc . [X, 1])^2,  Cc = (-src / SOD, SOD)
+//   matched: L^3 / (|X - src|^2 SDD_plane) * vx vy vz / (du dv),
+//            L^2 = |detS + u detU + v detV - src|^2.
+constexpr int kOpBpZ = 8;
+
+__device__ __forceinline__ float bilinear_view(const float* __restrict__ p, int nu, int nv, float u, float v)
+{
+    const float fu = floorf(u), fv = floorf(v);
+    const int iu = int(fu), iv = int(fv);
+    const float a = u - fu, b = v - fv;
+    const bool u0 = iu >= 0 && iu < nu, u1 = iu + 1 >= 0 && iu + 1 < nu;
+    const bool v0 = iv >= 0 && iv < nv, v1 = iv + 1 >= 0 && iv + 1 < nv;
+    const float t00 = (u0 && v0) ? p[(size_t)iv * nu + iu] : 0.f;
+    const float t01 = (u1 && v0) ? p[(size_t)iv * nu + iu + 1] : 0.f;
+    const float t10 = (u0 && v1) ? p[(size_t)(iv + 1) * nu + iu] : 0.f;
+    const float t11 = (u1 && v1) ? p[(size_t)(iv + 1) * nu + iu + 1] : 0.f;
+    return (1.f - b) * ((1.f - a) * t00 + a * t01) + b * ((1.f - a) * t10 + a * t11);
+}
+
+template <bool MATCHED>
+__global__ void __launch_bounds__(256)
+fdk_operator_bp_kernel(const float* __restrict__ proj, float* __restrict__ volume,
+    const OpBpViewDev* __restrict__ views, VolumeGeom g, int nu, int nv, int count, float scale,
+    int accumulate)
+{
+    const int x = blockIdx.x * 16 + int(threadIdx.x % 16);
+    const int y = blockIdx.y * 16 + int(threadIdx.x / 16);
+    const int z0 = blockIdx.z * kOpBpZ;
+    if (x >= g.nx || y >= g.ny) return;
+    const float X = g.ox + x * g.vx, Y = g.oy + y * g.vy;
+    float acc[kOpBpZ];
+#pragma unroll
+    for (int k = 0; k < kOpBpZ; ++k) acc[k] = 0.f;
+    for (int i = 0; i < count; ++i) {
+        const OpBpViewDev c = views[i];
+        const float* p = proj + (size_t)i * nu * nv;
+#pragma unroll
+        for (int k = 0; k < kOpBpZ; ++k) {
+            if (z0 + k >= g.nz) break;
+            const float Z = g.oz + (z0 + k) * g.vz;
+            const float den = c.cd[0] * X + c.cd[1] * Y + c.cd[2] * Z + c.cd[3];
+            if (!(den >= 1e-8f)) continue;
+            const float u = (c.cu[0] * X + c.cu[1] * Y + c.cu[2] * Z + c.cu[3]) / den;
+            const float v = (c.cv[0] * X + c.cv[1] * Y + c.cv[2] * Z + c.cv[3]) / den;
+            float w;
+            if constexpr (MATCHED) {
+                const float L2 = c.l2[0] + u * (c.l2[1] + u * c.l2[3]) + v * (c.l2[2] + u * c.l2[4] + v * c.l2[5]);
+                if (!(L2 >= 1e-8f)) continue;
+                const float dx = X - c.src[0], dy = Y - c.src[1], dz = Z - c.src[2];
+                const float lsq = dx * dx + dy * dy + dz * dz;
+                if (!(lsq >= 1e-8f)) continue;
+                w = L2 * sqrtf(L2) / lsq * c.inv_sdd_plane * scale;
+            } else {
+                const float cn = c.cc[0] * X + c.cc[1] * Y + c.cc[2] * Z + c.cc[3];
+                if (!(fabsf(cn) >= 1e-8f)) continue;
+                w = c.wbase / (cn * cn);
+            }
+            acc[k] += bilinear_view(p, nu, nv, u, v) * w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kOpBpZ; ++k) {
+        if (z0 + k >= g.nz) break;
+        float* dst = volume + ((size_t)(z0 + k) * g.ny + y) * g.nx + x;
+        *dst = accumulate ? *dst + acc[k] : acc[k];
+    }
+}
+
 }  // namespace
 
 void launch_joseph_fp(const JosephLaunch& p, hipStream_t s)
@@ -434,6 +507,18 @@ void launch_siddon_adjoint(const SiddonAdjointLaunch& p, hipStream_t s)
     dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, p.vol.nz);
     hipLaunchKernelGGL(siddon_adjoint_kernel, grid, block, 0, s, p.proj, p.volume, p.views, p.coeff64,
         p.vol, p.nu, p.nv, p.count, p.accumulate);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_fdk_operator_bp(const OpBpLaunch& p, hipStream_t s)
+{
+    dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, (p.vol.nz + kOpBpZ - 1) / kOpBpZ);
+    if (p.matched)
+        hipLaunchKernelGGL(fdk_operator_bp_kernel<true>, grid, block, 0, s, p.proj, p.volume, p.views, p.vol,
+            p.nu, p.nv, p.count, p.scale, p.accumulate);
+    else
+        hipLaunchKernelGGL(fdk_operator_bp_kernel<false>, grid, block, 0, s, p.proj, p.volume, p.views, p.vol,
+            p.nu, p.nv, p.count, p.scale, p.accumulate);
     YK_HIP(hipGetLastError());
 }
 

@@ -50,17 +50,49 @@ yk_projector::yk_projector(yk_ctx* c, const yk_view* v, int n, int nu_, int nv_,
         std::memcpy(s.detV, w.detV, sizeof(s.detV));
         projection_map(w, &c64[size_t(i) * 12]);
     }
+    // operator-form FDK backprojection records (FP64 geometry over the whole
+    // view sequence, like FdkAffineCoeff in .../FDK/kernels/YkFDKBpPrecompute.cu:14-138)
+    std::vector<DerivedD> der;
+    std::vector<CoeffD> cf;
+    derive_geometry(views, nu, nv, scan_range, der);
+    fdk_coefficients(views, der, cf);
+    std::vector<OpBpViewDev> ov(n);
+    op_scale.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const ViewD w = widen(views[i]);
+        OpBpViewDev& o = ov[i];
+        std::memset(&o, 0, sizeof(o));
+        for (int k = 0; k < 4; ++k) {
+            o.cu[k] = float(cf[i].cu[k]);
+            o.cv[k] = float(cf[i].cv[k]);
+            o.cd[k] = float(cf[i].cd[k]);
+        }
+        const double sod = der[i].sod;
+        for (int k = 0; k < 3; ++k) {
+            o.cc[k] = float(-w.src[k] / sod);
+            o.src[k] = float(w.src[k]);
+        }
+        o.cc[3] = float(sod);
+        double r0[3];
+        for (int k = 0; k < 3; ++k) r0[k] = w.detS[k] - w.src[k];
+        auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+        o.l2[0] = float(dot(r0, r0));
+        o.l2[1] = float(2.0 * dot(r0, w.detU));
+        o.l2[2] = float(2.0 * dot(r0, w.detV));
+        o.l2[3] = float(dot(w.detU, w.detU));
+        o.l2[4] = float(2.0 * dot(w.detU, w.detV));
+        o.l2[5] = float(dot(w.detV, w.detV));
+        o.wbase = float(cf[i].wbase);
+        o.inv_sdd_plane = std::fabs(der[i].sdd_plane) > 1e-12 ? float(1.0 / std::fabs(der[i].sdd_plane)) : 0.f;
+        op_scale[i] = vox[0] * vox[1] * vox[2] / der[i].du / der[i].dv;
+    }
     hipStream_t s = ctx->stream;
+    d_opviews.upload(ov.data(), ov.size(), s);
     d_jviews.upload(jv.data(), jv.size(), s);
     d_sviews.upload(sv.data(), sv.size(), s);
     d_c64.upload(c64.data(), c64.size(), s);
     d_partials.resize(1024);
     YK_HIP(hipStreamSynchronize(s));
-}
-
-yk_projector::~yk_projector()
-{
-    delete fdk_op;
 }
 
 void yk_projector::project(int kind, const float* d_volume, float* d_proj, int first, int count,
@@ -127,6 +159,17 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
         p.nu = nu; p.nv = nv; p.count = count;
         p.accumulate = accumulate;
         launch_siddon_adjoint(p, s);
+    } else if (kind == YK_BP_FDK || kind == YK_BP_FDK_PSEUDO_MATCHED) {
+        OpBpLaunch p{};
+        p.proj = d_proj;
+        p.volume = d_volume;
+        p.views = d_opviews.get() + first;
+        p.vol = vol;
+        p.nu = nu; p.nv = nv; p.count = count;
+        p.scale = float(op_scale[first]);
+        p.matched = kind == YK_BP_FDK_PSEUDO_MATCHED ? 1 : 0;
+        p.accumulate = accumulate;
+        launch_fdk_operator_bp(p, s);
     } else {
         fail(YK_ERR_UNSUPPORTED, "unknown back projector");
     }

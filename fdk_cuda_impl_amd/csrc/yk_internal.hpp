@@ -200,13 +200,13 @@ struct TileLaunch {
     int nbx, nby, nbz;          // block counts
     int nu, nv;
     short4* tiles;              // [n_blocks][n_views]: u0, v0, width, height (quads)
-    int* max_quads;             // device scalars: max width, max height, max w*h, bad
+    int* max_quads;             // device scalars: max width, max height, max staged quads (LDS), bad
 };
 void launch_bp_tiles(const TileLaunch& p, hipStream_t s);
 
 struct BpLaunch {
     const void* proj;           // [count][nv][nu], storage type
-    const float* row_scales;    // [count][nv] or null
+    const unsigned* row_factors; // [count][nv] normalized row factors (half / bf16) or null
     const int* gscale_bits;     // device float bits: max row scale (half / bf16)
     const BpCoeff* coeffs;      // device, `count` records
     const short4* tiles;        // [n_blocks][n_tile_views] with column offset
@@ -224,7 +224,10 @@ struct BpLaunch {
     int lds_quads;              // LDS capacity in quads
 };
 void launch_fdk_bp(const BpLaunch& p, hipStream_t s);
-void launch_max_scale(const float* scales, int n, int* out_bits, hipStream_t s);
+// global maximum row scale into *gbits, then per row the factor scale / max
+// (packed half pair for half storage, float bits for bfloat16)
+void launch_row_factors(const float* scales, int n, int storage, int* gbits, unsigned* factors,
+    hipStream_t s);
 int bp_block_z(int general);
 
 struct ProjViewDev {
@@ -286,6 +289,28 @@ struct SiddonAdjointLaunch {
     int accumulate;
 };
 void launch_siddon_adjoint(const SiddonAdjointLaunch& p, hipStream_t s);
+
+// per-view record of the operator-form FDK backprojections (world units)
+struct OpBpViewDev {
+    float cu[4], cv[4], cd[4];  // projective map u = Cu.[X,1] / Cd.[X,1], v likewise
+    float cc[4];                // FDK weight denominator (-src / SOD, SOD)
+    float src[4];
+    float l2[6];                // |detS + u detU + v detV - src|^2: 1, u, v, uu, uv, vv
+    float wbase;                // SOD^2 dtheta fscale
+    float inv_sdd_plane;
+};
+
+struct OpBpLaunch {
+    const float* proj;          // [count][nv][nu], unfiltered
+    float* volume;              // [z][y][x]
+    const OpBpViewDev* views;   // device, `count`
+    VolumeGeom vol;
+    int nu, nv, count;
+    float scale;                // vx vy vz / (du dv) of the first view (matched weight)
+    int matched;
+    int accumulate;
+};
+void launch_fdk_operator_bp(const OpBpLaunch& p, hipStream_t s);
 
 // vector helpers
 void launch_dot(const float* a, const float* b, size_t n, double* partials, int n_partials,

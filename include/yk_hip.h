@@ -74,8 +74,13 @@ typedef enum yk_parker_mode {
 /* forward models, YK::ETask FP_* (src/common/YkOperatorTypes.hpp:5-22) */
 typedef enum yk_forward_kind { YK_FP_JOSEPH = 1, YK_FP_SIDDON = 2 } yk_forward_kind;
 
-/* back models: exact transposes of the forward models, and the FDK
- * operator-form backprojectors used as B by SART (YkBpFdkKernel.cu) */
+/* back models: exact transposes of the forward models, and the operator-form
+ * FDK backprojectors of unfiltered views used as B by the iterative solvers
+ * (fdk_bp_launch / fdk_pseudo_matched_bp_launch,
+ * src/FlatFpBp/BP/kernels/YkBpFdkKernel.cu:228-557): weight
+ * SOD^2 dtheta fscale / (SOD - src.X / SOD)^2, or the pseudo-matched
+ * L^3 / (|X - src|^2 SDD_plane) * vx vy vz / (du dv).  Derived geometry
+ * (dtheta) is taken over the projector's whole view sequence. */
 typedef enum yk_back_kind {
     YK_BP_JOSEPH_ADJOINT = 8, YK_BP_SIDDON_ADJOINT = 4,
     YK_BP_FDK = 6, YK_BP_FDK_PSEUDO_MATCHED = 7

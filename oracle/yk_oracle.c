@@ -718,6 +718,65 @@ void yko_bp_fdk(const double* coeffs, int n_views, const double* filtered,
     }
 }
 
+/*
+ * Operator-form FDK backprojections used as the iterative B
+ * (fdk_bp_kernel_fdk_weight_v2 / fdk_bp_kernel_matched_weight,
+ * src/FlatFpBp/BP/kernels/YkBpFdkKernel.cu:228-466), on unfiltered views:
+ *   FDK:     w = SOD^2 dtheta fscale / (SOD - src.X / SOD)^2
+ *   matched: w = L^3 / (|X - src|^2 |SDD_plane|) * vx vy vz / (du0 dv0),
+ *            L = |detS + u detU + v detV - src|, du0/dv0 of the first view
+ * A sample is skipped when den < 1e-8, |SOD - src.X/SOD| < 1e-8, L^2 < 1e-8
+ * or |X - src|^2 < 1e-8.  out ([nz][ny][nx]) is overwritten.
+ */
+void yko_bp_fdk_operator(const double* views, const double* derived, const double* coeffs,
+    int n_views, const double* proj, int nu, int nv, const yko_volume* vol, int matched,
+    double* out)
+{
+    const double ox = vol->cx - (vol->nx - 1) * 0.5 * vol->vx;
+    const double oy = vol->cy - (vol->ny - 1) * 0.5 * vol->vy;
+    const double oz = vol->cz - (vol->nz - 1) * 0.5 * vol->vz;
+    const size_t plane = (size_t)nu * nv;
+    const double scale = vol->vx * vol->vy * vol->vz /
+        (derived[YKO_D_DU] * derived[YKO_D_DV]);
+    const int64_t total = (int64_t)vol->nx * vol->ny * vol->nz;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t k = 0; k < total; ++k) {
+        const int64_t x = k % vol->nx;
+        const int64_t y = (k / vol->nx) % vol->ny;
+        const int64_t z = k / ((int64_t)vol->nx * vol->ny);
+        const double X[3] = { ox + x * vol->vx, oy + y * vol->vy, oz + z * vol->vz };
+        double acc = 0.0;
+        for (int i = 0; i < n_views; ++i) {
+            const double* g = views + (size_t)i * YKO_VIEW_STRIDE;
+            const double* d = derived + (size_t)i * YKO_DERIVED_STRIDE;
+            const double* c = coeffs + (size_t)i * YKO_COEFF_STRIDE;
+            const double den = c[YKO_C_CD + 3] + X[0] * c[YKO_C_CD] + X[1] * c[YKO_C_CD + 1] + X[2] * c[YKO_C_CD + 2];
+            if (!(den >= 1e-8)) continue;
+            const double u = (c[YKO_C_CU + 3] + X[0] * c[YKO_C_CU] + X[1] * c[YKO_C_CU + 1] + X[2] * c[YKO_C_CU + 2]) / den;
+            const double v = (c[YKO_C_CV + 3] + X[0] * c[YKO_C_CV] + X[1] * c[YKO_C_CV + 1] + X[2] * c[YKO_C_CV + 2]) / den;
+            double w;
+            if (matched) {
+                double P[3], L2 = 0.0, lsq = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    P[a] = g[3 + a] + u * g[6 + a] + v * g[9 + a] - g[a];
+                    L2 += P[a] * P[a];
+                    lsq += (X[a] - g[a]) * (X[a] - g[a]);
+                }
+                if (!(L2 >= 1e-8) || !(lsq >= 1e-8)) continue;
+                const double sdd = fabs(d[YKO_D_SDD_PLANE]);
+                w = L2 * sqrt(L2) / lsq * (sdd > 1e-12 ? 1.0 / sdd : 0.0) * scale;
+            } else {
+                const double sod = d[YKO_D_SOD];
+                const double cn = sod - (g[0] * X[0] + g[1] * X[1] + g[2] * X[2]) / sod;
+                if (!(fabs(cn) >= 1e-8)) continue;
+                w = sod * sod * d[YKO_D_DTHETA] * d[YKO_D_FSCALE] / (cn * cn);
+            }
+            acc += yko_bilinear(proj + plane * i, nu, nv, u, v) * w;
+        }
+        out[k] = acc;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* forward projectors                                                        */
 /* ------------------------------------------------------------------------ */

@@ -106,6 +106,11 @@ double yko_bilinear(const double* plane, int nu, int nv, double u, double v);
 void yko_bp_fdk(const double* coeffs, int n_views, const double* filtered,
     int nu, int nv, const yko_volume* vol, const int64_t* voxel_index,
     int64_t n_voxels, double* out);
+/* operator-form FDK backprojections (weight FDK or pseudo-matched) of
+ * unfiltered views (src/FlatFpBp/BP/kernels/YkBpFdkKernel.cu:228-466) */
+void yko_bp_fdk_operator(const double* views, const double* derived, const double* coeffs,
+    int n_views, const double* proj, int nu, int nv, const yko_volume* vol, int matched,
+    double* out);
 
 /* ---- forward projectors and transposes -------------------------------- */
 int yko_main_axis(const double* view, int nu, int nv);

@@ -78,6 +78,7 @@ def lib():
             "yko_bilinear": (c_double, [dp, c_int, c_int, c_double, c_double]),
             "yko_bp_fdk": (None, [dp, c_int, dp, c_int, c_int, vp, i64p, ctypes.c_int64, dp]),
             "yko_main_axis": (c_int, [dp, c_int, c_int]),
+            "yko_bp_fdk_operator": (None, [dp, dp, dp, c_int, dp, c_int, c_int, vp, c_int, dp]),
             "yko_fp_joseph": (None, [dp, c_int, c_int, c_int, vp, fp, dp]),
             "yko_bp_joseph_adjoint": (None, [dp, c_int, c_int, c_int, vp, dp, dp]),
             "yko_fp_siddon": (None, [dp, c_int, c_int, c_int, vp, fp, dp]),
@@ -234,6 +235,20 @@ def bp_fdk(coeffs_, filtered, vol: Volume, voxel_index=None) -> np.ndarray:
         out = np.zeros(idx.size)
         lib().yko_bp_fdk(_dp(c), n, _dp(f), nu, nv, ctypes.byref(vol),
                          idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), idx.size, _dp(out))
+    return out
+
+
+def bp_fdk_operator(views, proj, vol: Volume, matched=False, scan_range=None) -> np.ndarray:
+    """Operator-form FDK backprojection of unfiltered views (weight FDK or
+    pseudo-matched), derived geometry over the whole view sequence."""
+    views = _d(views)
+    p = _d(proj)
+    n, nv, nu = p.shape
+    d = derive(views, nu, nv, 2.0 * np.pi if scan_range is None else scan_range)
+    c = coeffs(views, d)
+    out = np.zeros((vol.nz, vol.ny, vol.nx))
+    lib().yko_bp_fdk_operator(_dp(views), _dp(d), _dp(c), n, _dp(p), nu, nv, ctypes.byref(vol),
+                              1 if matched else 0, _dp(out))
     return out
 
 

@@ -560,3 +560,49 @@ def test_tv_value_and_gradient(oracle):
         xm[k] -= h
         fd = (oracle.tv_value_grad(xp, vol, 0.1)[0] - oracle.tv_value_grad(xm, vol, 0.1)[0]) / (2 * h)
         assert abs(fd - g[k]) < 1e-6 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("matched", [False, True])
+def test_bp_fdk_operator_brute_force(oracle, matched):
+    """Operator-form FDK backprojections (src/FlatFpBp/BP/kernels/YkBpFdkKernel.cu:228-466)
+    with a source above the orbit plane, so the FDK weight's denominator
+    SOD - src.X / SOD differs from the radial depth of the FDK kernel."""
+    n, nu, nv = 6, 18, 14
+    v = oracle.build_views(n, n, 60.0, 110.0, nu, nv, 1.2, 1.1, offset_unv=(0.5, 0.0, 0.4),
+                           source_offset=(0.0, 0.0, 3.0), tilt=(0.03, 0.0, -0.02))
+    vol = oracle.volume(5, 4, 3, 1.5, 1.4, 1.3, center=(0.5, -0.3, 0.2))
+    p = np.random.default_rng(12).standard_normal((n, nv, nu))
+    got = oracle.bp_fdk_operator(v, p, vol, matched=matched)
+    du, dv = np.linalg.norm(v[0, 6:9]), np.linalg.norm(v[0, 9:12])
+    want = np.zeros((3, 4, 5))
+    for z in range(3):
+        for y in range(4):
+            for x in range(5):
+                X = np.array([0.5 + (x - 2) * 1.5, -0.3 + (y - 1.5) * 1.4, 0.2 + (z - 1) * 1.3])
+                acc = 0.0
+                for i, g in enumerate(v):
+                    src = g[0:3]
+                    u, vv = project_point(g, X)
+                    if matched:
+                        L = np.linalg.norm(g[3:6] + u * g[6:9] + vv * g[9:12] - src)
+                        m = np.cross(g[6:9], g[9:12])
+                        sdd = abs(np.dot(g[3:6] - src, m / np.linalg.norm(m)))
+                        w = L ** 3 / np.dot(X - src, X - src) / sdd * (1.5 * 1.4 * 1.3) / (du * dv)
+                    else:
+                        sod = np.hypot(src[0], src[1])
+                        w = sod * sod * (2 * np.pi / n) / (sod - np.dot(src, X) / sod) ** 2
+                    acc += interp2(p[i], u, vv)[0] * w
+                want[z, y, x] = acc
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12)
+
+
+def test_bp_fdk_operator_equals_fdk_bp_in_the_orbit_plane(oracle):
+    """With the source in z = 0 the FDK weight denominator is the radial depth,
+    so the operator equals the FDK kernel's backprojection of the same data."""
+    n, nu, nv = 8, 20, 16
+    v = oracle.build_views(n, n, 60.0, 110.0, nu, nv, 1.2, 1.1, offset_unv=(0.5, 0.0, 0.4))
+    vol = oracle.volume(6, 5, 4, 1.5, 1.4, 1.3)
+    p = np.random.default_rng(2).standard_normal((n, nv, nu))
+    d = oracle.derive(v, nu, nv, 2 * np.pi)
+    np.testing.assert_allclose(oracle.bp_fdk_operator(v, p, vol), oracle.bp_fdk(oracle.coeffs(v, d), p, vol),
+                               rtol=1e-12, atol=1e-14)
