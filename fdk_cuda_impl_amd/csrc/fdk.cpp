@@ -188,6 +188,7 @@ void yk_fdk::build_tiles()
         nbz = (grid.nz + bz - 1) / bz;
         const size_t entries = size_t(nbx) * nby * nbz * n_views;
         d_tiles.resize(entries);
+        d_stage.resize(entries);
         d_maxes.resize(8);
         TileLaunch t{};
         t.coeff64 = d_c64.get();
@@ -197,6 +198,7 @@ void yk_fdk::build_tiles()
         t.nbx = nbx; t.nby = nby; t.nbz = nbz;
         t.nu = nu; t.nv = nv;
         t.tiles = d_tiles.get();
+        t.stage = d_stage.get();
         t.max_quads = d_maxes.get();
         launch_bp_tiles(t, s);
         int maxes[4] = { 0, 0, 0, 0 };
@@ -249,6 +251,7 @@ void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_sca
     p.gscale_bits = d_maxes.get() + 4;
     p.coeffs = d_coeffs.get() + first;
     p.tiles = d_tiles.get();
+    p.stage = d_stage.get();
     p.tile_view_stride = n_views;
     p.tile_view_offset = first;
     p.volume = d_volume;

@@ -36,6 +36,7 @@ struct yk_fdk {
     yk::DeviceArray<double> d_c64;
     yk::DeviceArray<float> d_parker;
     yk::DeviceArray<short4> d_tiles;
+    yk::DeviceArray<uint2> d_stage;
     yk::DeviceArray<int> d_maxes;      // [0..3] footprint maxima, [4] global row-scale bits
     yk::DeviceArray<unsigned> d_rowfac; // normalized row factors of the views being backprojected
     int nbx = 0, nby = 0, nbz = 0, bz = 0;

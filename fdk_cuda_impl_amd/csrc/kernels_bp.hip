@@ -82,7 +82,7 @@ __host__ __device__ __forceinline__ int stage_quads(int qw, int qh)
 // ---------------------------------------------------------------------------
 __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, VolumeGeom vol,
     int bx, int by, int bz, int nbx, int nby, int nbz, int nu, int nv,
-    short4* __restrict__ tiles, int* __restrict__ maxes)
+    short4* __restrict__ tiles, uint2* __restrict__ stage, int* __restrict__ maxes)
 {
     const long long total = (long long)nbx * nby * nbz * n_views;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,6 +111,7 @@ __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, Vol
         vmin = fmin(vmin, v); vmax = fmax(vmax, v);
     }
     short4 t = make_short4(0, 0, 0, 0);
+    uint2 st = make_uint2(0u, 0u);
     if (!bad) {
         const double fu0 = floor(umin) - 1.0, fu1 = floor(umax) + 1.0;
         const double fv0 = floor(vmin) - 1.0, fv1 = floor(vmax) + 1.0;
@@ -128,11 +129,17 @@ __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, Vol
                 atomicMax(&maxes[0], int(t.z));
                 atomicMax(&maxes[1], int(t.w));
                 atomicMax(&maxes[2], stage_quads(t.z, t.w));
+                // staging shape, so the backprojection divides nothing per view:
+                // x = 16-bit reciprocal of the span, y = chunks | rows_per << 16
+                const StageShape sh = stage_shape(t.z, t.w);
+                st.x = (65536u + unsigned(sh.span) - 1u) / unsigned(sh.span);
+                st.y = unsigned(sh.chunks) | (unsigned(min(sh.rows_per, 65535)) << 16);
             }
         }
     }
     if (bad) atomicMax(&maxes[3], 1);
     tiles[idx] = t;
+    stage[idx] = st;
 }
 
 // small fill used instead of hipMemsetAsync so that callers may capture the
@@ -196,14 +203,15 @@ struct StageMap {
     bool fast;    // tile fits the register prefetch
 };
 
-__device__ __forceinline__ StageMap stage_map(short4 t, int nu)
+__device__ __forceinline__ StageMap stage_map(short4 t, uint2 st, int nu)
 {
     StageMap m;
-    m.s = stage_shape(t.z, t.w);
-    // threadIdx.x / span by a 16-bit reciprocal: exact for threadIdx.x < 256
-    // and span <= 256 (the rounding excess stays below 1/256)
-    const unsigned magic = (65536u + unsigned(m.s.span) - 1u) / unsigned(m.s.span);
-    const int rc = int((threadIdx.x * magic) >> 16);
+    m.s.span = min(int(t.z), kThreads);
+    m.s.chunks = int(st.y & 0xffffu);
+    m.s.rows_per = __builtin_amdgcn_readfirstlane(int(st.y >> 16));
+    // threadIdx.x / span by the 16-bit reciprocal st.x: exact for
+    // threadIdx.x < 256 and span <= 256 (the rounding excess stays below 1/256)
+    const int rc = int((threadIdx.x * st.x) >> 16);
     m.col = int(threadIdx.x) - rc * m.s.span;
     const int col = t.x + m.col;
     m.cl = min(max(col, 0), nu - 2);
@@ -483,7 +491,7 @@ template <int STORAGE, int BZ, bool GENERAL, bool KAHAN>
 __global__ void __launch_bounds__(kThreads)
 fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_factors,
     const int* __restrict__ gscale_bits, const BpCoeff* __restrict__ coeffs,
-    const short4* __restrict__ tiles, int tile_stride, int tile_offset,
+    const short4* __restrict__ tiles, const uint2* __restrict__ stage, int tile_stride, int tile_offset,
     float* __restrict__ volume, VolumeGeom vol, int nu, int nv, int count, int accumulate)
 {
     extern __shared__ float4 lds_raw[];
@@ -498,6 +506,7 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
     const float Z0 = vol.oz + z0 * vol.vz;
     const size_t block = ((size_t)bzi * gridDim.y + byi) * gridDim.x + bxi;
     const short4* my_tiles = tiles + block * tile_stride + tile_offset;
+    const uint2* my_stage = stage + block * tile_stride + tile_offset;
     const unsigned view_bytes = unsigned(nu) * unsigned(nv) * (STORAGE == YK_F32 ? 4u : 2u);
     auto data_rsrc = [&](int i) {
         return make_rsrc(static_cast<const char*>(proj) + (size_t)view_bytes * i, view_bytes);
@@ -531,7 +540,7 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
     Prefetch<STORAGE> pf;
     short4 t_cur = my_tiles[0];
     if (t_cur.z > 0) {
-        const StageMap m = stage_map(t_cur, nu);
+        const StageMap m = stage_map(t_cur, my_stage[0], nu);
         if (m.fast) {
             prefetch_tile<STORAGE>(pf, m, data_rsrc(0), fac_rsrc(0), nu, t_cur);
             commit_tile<STORAGE>(lds, pf, m, t_cur);
@@ -546,7 +555,7 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
         const short4 t_next = (i + 1 < count) ? my_tiles[i + 1] : make_short4(0, 0, 0, 0);
         StageMap m_next{};
         if (t_next.z > 0) {
-            m_next = stage_map(t_next, nu);
+            m_next = stage_map(t_next, my_stage[i + 1], nu);
             if (m_next.fast)
                 prefetch_tile<STORAGE>(pf, m_next, data_rsrc(i + 1), fac_rsrc(i + 1), nu, t_next);
         }
@@ -602,7 +611,7 @@ void launch_bz(const BpLaunch& p, const int* gbits, hipStream_t s)
     auto k = fdk_bp_kernel<STORAGE, BZ, GENERAL, KAHAN>;
     YK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL(k, grid, block, lds, s, p.proj, p.row_factors, gbits, p.coeffs, p.tiles,
+    hipLaunchKernelGGL(k, grid, block, lds, s, p.proj, p.row_factors, gbits, p.coeffs, p.tiles, p.stage,
         p.tile_view_stride, p.tile_view_offset, p.volume, p.vol, p.nu, p.nv, p.count, p.accumulate);
 }
 
@@ -639,7 +648,7 @@ void launch_bp_tiles(const TileLaunch& p, hipStream_t s)
     if (blocks > 0x7fffffffLL) fail(YK_ERR_UNSUPPORTED, "footprint table too large");
     hipLaunchKernelGGL(set_ints_kernel, dim3(1), dim3(64), 0, s, p.max_quads, 4, 0);
     hipLaunchKernelGGL(bp_tiles_kernel, dim3(unsigned(blocks)), dim3(threads), 0, s, p.coeff64,
-        p.n_views, p.vol, p.bx, p.by, p.bz, p.nbx, p.nby, p.nbz, p.nu, p.nv, p.tiles, p.max_quads);
+        p.n_views, p.vol, p.bx, p.by, p.bz, p.nbx, p.nby, p.nbz, p.nu, p.nv, p.tiles, p.stage, p.max_quads);
     YK_HIP(hipGetLastError());
 }
 

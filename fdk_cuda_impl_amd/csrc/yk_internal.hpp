@@ -200,6 +200,7 @@ struct TileLaunch {
     int nbx, nby, nbz;          // block counts
     int nu, nv;
     short4* tiles;              // [n_blocks][n_views]: u0, v0, width, height (quads)
+    uint2* stage;               // [n_blocks][n_views]: staging shape (span reciprocal, chunks, rows)
     int* max_quads;             // device scalars: max width, max height, max staged quads (LDS), bad
 };
 void launch_bp_tiles(const TileLaunch& p, hipStream_t s);
@@ -210,6 +211,7 @@ struct BpLaunch {
     const int* gscale_bits;     // device float bits: max row scale (half / bf16)
     const BpCoeff* coeffs;      // device, `count` records
     const short4* tiles;        // [n_blocks][n_tile_views] with column offset
+    const uint2* stage;         // staging shapes, same indexing as tiles
     int tile_view_stride;       // = total views of the plan
     int tile_view_offset;       // first view in the tile table
     float* volume;
