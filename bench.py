@@ -40,7 +40,22 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle accuracy/CPU leg")
     ap.add_argument("--size", type=int, default=512, help="volume edge (default 512)")
     ap.add_argument("--views", type=int, default=720)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one 1 GiB device copy (reference dispatch for rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE scaling)")
     return ap.parse_args()
+
+
+def measured_traffic():
+    """HBM bytes per FP16 backprojection launch at this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated);
+    None when the summary is absent."""
+    path = os.path.join(REPO, "profiles", "fdk_bp_512cube_720views_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["fdk_bp_f16_kahan"]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def time_launches(fn, steps, warmup, stream):
@@ -107,6 +122,11 @@ def main():
     from fdk_cuda_impl_amd import phantoms
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.ones(256 * 1024 * 1024, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     s = args.size
     vox = 0.25 * 512 / s
     geom = yk.ScanGeometry(args.views, args.views, 600.0, 1100.0, 1024, 768, 0.3, 0.3,
@@ -157,7 +177,8 @@ def main():
                    "storage": "fp16 rows + per-row fp32 scales, fp32 Kahan accumulation",
                    "views_per_launch": args.views},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": measured_traffic(),
                      "note": f"VALU/LDS-bound kernel; {FLOP_PER_UPDATE['f16_kahan']} algorithmic FLOP per "
                              f"update; HBM {alg_bytes16 / (ms16 * 1e-3) / 1e9:.0f} GB/s algorithmic"},
         "fp32_gups": round(gups32, 2), "fp32_ms_per_step": round(ms32, 3),

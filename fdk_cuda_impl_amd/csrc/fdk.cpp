@@ -182,7 +182,8 @@ void yk_fdk::build_tiles()
     hipStream_t s = ctx->stream;
     const size_t max_lds = 96 * 1024;
     const size_t quad = desc.storage == YK_F32 ? 16 : 8;
-    for (bz = bp_block_z(general ? 1 : 0); bz >= 8; bz /= 2) {
+    const int first_bz = bp_block_z(general ? 1 : 0, desc.storage, desc.kahan);
+    for (bz = first_bz; bz >= 8; bz = bz > 16 ? 16 : bz / 2) {
         nbx = (grid.nx + 15) / 16;
         nby = (grid.ny + 15) / 16;
         nbz = (grid.nz + bz - 1) / bz;

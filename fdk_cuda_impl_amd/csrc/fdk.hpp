@@ -71,6 +71,9 @@ struct yk_projector {
     yk::DeviceArray<yk::ProjViewDev> d_jviews;
     yk::DeviceArray<yk::SiddonViewDev> d_sviews;
     yk::DeviceArray<double> d_c64;
+    yk::DeviceArray<float> d_c32;      // the projective map in float (Joseph transpose candidates)
+    yk::DeviceArray<float4> d_ray_ab;  // Joseph transpose ray table, one view chunk
+    yk::DeviceArray<float> d_ray_len;
     yk::DeviceArray<float> d_vol_t;
     yk::DeviceArray<double> d_partials;
     yk::DeviceArray<yk::OpBpViewDev> d_opviews;  // operator-form FDK backprojection records

@@ -43,7 +43,7 @@ constexpr int kBX = 16;
 constexpr int kBY = 16;
 constexpr int kThreads = kBX * kBY;
 constexpr int kKahanViews = 16;
-constexpr int kMaxRows = 16;  // texel rows a thread prefetches (quad rows + 1)
+constexpr int kMaxRows = 12;  // texel rows a thread prefetches (quad rows + 1)
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
@@ -487,18 +487,34 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
     }
 }
 
+// workgroups per CU the register allocation must leave room for: four (128
+// VGPRs) for the half / bfloat16 Kahan variant, whose state would otherwise
+// just exceed that budget; the others fit it or need more on their own
+constexpr int bp_min_blocks(int storage, int bz, bool kahan)
+{
+    return (storage != YK_F32 && kahan && bz == 32) ? 4 : 1;
+}
+
 template <int STORAGE, int BZ, bool GENERAL, bool KAHAN>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, bp_min_blocks(STORAGE, BZ, KAHAN))
 fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_factors,
     const int* __restrict__ gscale_bits, const BpCoeff* __restrict__ coeffs,
     const short4* __restrict__ tiles, const uint2* __restrict__ stage, int tile_stride, int tile_offset,
-    float* __restrict__ volume, VolumeGeom vol, int nu, int nv, int count, int accumulate)
+    float* __restrict__ volume, VolumeGeom vol, int nu, int nv, int count, int accumulate,
+    unsigned buf_bytes, int double_buffer)
 {
+    // one LDS tile buffer, or two alternating ones (then a single barrier per
+    // view orders the writes of view i+1 after every read of view i-1)
     extern __shared__ float4 lds_raw[];
-    void* lds = lds_raw;
+    char* const lds0 = reinterpret_cast<char*>(lds_raw);
+    unsigned cur = 0, nxt = double_buffer ? buf_bytes : 0u;
     const int bxi = blockIdx.x, byi = blockIdx.y, bzi = blockIdx.z;
-    const int x = bxi * kBX + (threadIdx.x % kBX);
-    const int y = byi * kBY + (threadIdx.x / kBX);
+    // lane -> column: each wave covers 16 x 4 columns, and every 8 consecutive
+    // lanes a 4 x 2 patch, so the lanes one LDS cycle serves read nearby quads
+    // of distinct banks whatever the view angle
+    const int l = int(threadIdx.x) & 63;
+    const int x = bxi * kBX + ((l & 3) | (((l >> 3) & 3) << 2));
+    const int y = byi * kBY + 4 * (int(threadIdx.x) >> 6) + (((l >> 2) & 1) | ((l >> 5) << 1));
     const int z0 = bzi * BZ;
     const bool active = x < vol.nx && y < vol.ny;
     const float X = vol.ox + x * vol.vx;
@@ -524,16 +540,14 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
     float acc[BZ];
 #pragma unroll
     for (int k = 0; k < BZ; ++k) acc[k] = 0.f;
-    // Kahan state: FP32 sums, compensations truncated to bfloat16 and packed
-    // two per register (a compensation is the rounding error of its sum, so
-    // 8 bits of it leave the compensated sum within ~2^-8 ulp)
+    // Kahan summation over blocks of kKahanViews views: acc holds the block
+    // partial, sum the running total; the rounding error of each sum += acc is
+    // carried into the next block by starting acc at minus that error (the
+    // compensation of Kahan's y = x - c, kept in acc instead of a third array)
     float sum[KAHAN ? BZ : 1];
-    unsigned comp[KAHAN ? BZ / 2 : 1];
     if constexpr (KAHAN) {
 #pragma unroll
         for (int k = 0; k < BZ; ++k) sum[k] = 0.f;
-#pragma unroll
-        for (int k = 0; k < BZ / 2; ++k) comp[k] = 0u;
     }
 
     // stage view 0
@@ -543,9 +557,9 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
         const StageMap m = stage_map(t_cur, my_stage[0], nu);
         if (m.fast) {
             prefetch_tile<STORAGE>(pf, m, data_rsrc(0), fac_rsrc(0), nu, t_cur);
-            commit_tile<STORAGE>(lds, pf, m, t_cur);
+            commit_tile<STORAGE>(lds0, pf, m, t_cur);
         } else {
-            stage_direct<STORAGE>(lds, data_rsrc(0), fac_rsrc(0), nu, t_cur);
+            stage_direct<STORAGE>(lds0, data_rsrc(0), fac_rsrc(0), nu, t_cur);
         }
     }
     __syncthreads();
@@ -560,29 +574,25 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
                 prefetch_tile<STORAGE>(pf, m_next, data_rsrc(i + 1), fac_rsrc(i + 1), nu, t_next);
         }
         if (t_cur.z > 0 && active)
-            accumulate_view<STORAGE, BZ, GENERAL>(lds, coeffs[i], t_cur, X, Y, Z0, vol.vz, acc);
+            accumulate_view<STORAGE, BZ, GENERAL>(lds0 + cur, coeffs[i], t_cur, X, Y, Z0, vol.vz, acc);
         if constexpr (KAHAN) {
             if ((i + 1) % kKahanViews == 0 || i + 1 == count) {
 #pragma unroll
-                for (int k = 0; k < BZ; k += 2) {
-                    const float c0 = __uint_as_float(comp[k / 2] << 16);
-                    const float c1 = __uint_as_float(comp[k / 2] & 0xffff0000u);
-                    const float y0 = acc[k] - c0, y1 = acc[k + 1] - c1;
-                    const float t0 = sum[k] + y0, t1 = sum[k + 1] + y1;
-                    const float n0 = (t0 - sum[k]) - y0, n1 = (t1 - sum[k + 1]) - y1;
-                    comp[k / 2] = (__float_as_uint(n0) >> 16) | (__float_as_uint(n1) & 0xffff0000u);
-                    sum[k] = t0;
-                    sum[k + 1] = t1;
-                    acc[k] = 0.f;
-                    acc[k + 1] = 0.f;
+                for (int k = 0; k < BZ; ++k) {
+                    const float t = sum[k] + acc[k];
+                    acc[k] = acc[k] - (t - sum[k]);  // what t lost of acc
+                    sum[k] = t;
                 }
             }
         }
         if (t_next.z > 0) {
+            if (!double_buffer) __syncthreads();
+            if (m_next.fast) commit_tile<STORAGE>(lds0 + nxt, pf, m_next, t_next);
+            else stage_direct<STORAGE>(lds0 + nxt, data_rsrc(i + 1), fac_rsrc(i + 1), nu, t_next);
             __syncthreads();
-            if (m_next.fast) commit_tile<STORAGE>(lds, pf, m_next, t_next);
-            else stage_direct<STORAGE>(lds, data_rsrc(i + 1), fac_rsrc(i + 1), nu, t_next);
-            __syncthreads();
+            const unsigned t = cur;
+            cur = nxt;
+            nxt = t;
         }
         t_cur = t_next;
     }
@@ -594,8 +604,7 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
         const int z = z0 + k;
         if (z >= vol.nz) break;
         float val;
-        if constexpr (KAHAN)
-            val = sum[k] - __uint_as_float((k & 1) ? (comp[k / 2] & 0xffff0000u) : (comp[k / 2] << 16));
+        if constexpr (KAHAN) val = sum[k] + acc[k];
         else val = acc[k];
         val *= g;
         float* dst = volume + (size_t)z * plane + col;
@@ -607,18 +616,30 @@ template <int STORAGE, int BZ, bool GENERAL, bool KAHAN>
 void launch_bz(const BpLaunch& p, const int* gbits, hipStream_t s)
 {
     dim3 grid(p.nbx, p.nby, p.nbz), block(kThreads);
-    const size_t lds = size_t(p.lds_quads) * (STORAGE == YK_F32 ? 16 : 8);
+    const size_t buf = size_t(p.lds_quads) * (STORAGE == YK_F32 ? 16 : 8);
     auto k = fdk_bp_kernel<STORAGE, BZ, GENERAL, KAHAN>;
+    // double-buffer the tile when that costs no occupancy: workgroups per CU
+    // allowed by the VGPRs (one wave per SIMD each) vs by 160 KB of LDS
+    hipFuncAttributes attr{};
+    YK_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(k)));
+    const int vgprs = std::max(8, (attr.numRegs + 7) / 8 * 8);
+    const size_t lds_cu = 160 * 1024;
+    const int wg_vgpr = std::min(8, 512 / vgprs);
+    const int wg_db = int(lds_cu / std::max<size_t>(1, 2 * buf));
+    int db = (wg_db >= wg_vgpr && 2 * buf <= 64 * 1024) ? 1 : 0;
+    const size_t lds = db ? 2 * buf : buf;
     YK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     hipLaunchKernelGGL(k, grid, block, lds, s, p.proj, p.row_factors, gbits, p.coeffs, p.tiles, p.stage,
-        p.tile_view_stride, p.tile_view_offset, p.volume, p.vol, p.nu, p.nv, p.count, p.accumulate);
+        p.tile_view_stride, p.tile_view_offset, p.volume, p.vol, p.nu, p.nv, p.count, p.accumulate,
+        unsigned(buf), db);
 }
 
 template <int STORAGE, bool GENERAL, bool KAHAN>
 void launch_variant(const BpLaunch& p, const int* gbits, hipStream_t s, int bz)
 {
     if (bz == 32) launch_bz<STORAGE, 32, GENERAL, KAHAN>(p, gbits, s);
+    else if (bz == 24) launch_bz<STORAGE, 24, GENERAL, KAHAN>(p, gbits, s);
     else if (bz == 16) launch_bz<STORAGE, 16, GENERAL, KAHAN>(p, gbits, s);
     else if (bz == 8) launch_bz<STORAGE, 8, GENERAL, KAHAN>(p, gbits, s);
     else fail(YK_ERR_UNSUPPORTED, "unsupported block depth");
@@ -638,7 +659,11 @@ void launch_storage(const BpLaunch& p, const int* gbits, hipStream_t s, int bz)
 
 }  // namespace
 
-int bp_block_z(int general) { return general ? 16 : 32; }
+int bp_block_z(int general, int storage, int kahan)
+{
+    (void)storage; (void)kahan;
+    return general ? 16 : 32;
+}
 
 void launch_bp_tiles(const TileLaunch& p, hipStream_t s)
 {

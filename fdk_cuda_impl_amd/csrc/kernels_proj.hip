@@ -178,10 +178,39 @@ __device__ __forceinline__ bool project64(const double* c, double X, double Y, d
     return true;
 }
 
-// Joseph transpose: one thread per voxel, gather over views and candidate rays
+// Ray table of the Joseph transpose: the parameters of every ray of `count`
+// views, from the same joseph_ray routine the forward kernel calls (so the
+// transpose reuses bit-identical lines); L = 0 marks a ray the forward skips.
+__global__ void __launch_bounds__(256)
+joseph_ray_table_kernel(const ProjViewDev* __restrict__ views, VolumeGeom g, int nu, int nv,
+    float4* __restrict__ ab, float* __restrict__ len)
+{
+    const int u = blockIdx.x * 64 + threadIdx.x;
+    const int v = blockIdx.y * 4 + threadIdx.y;
+    const int i = blockIdx.z;
+    if (u >= nu || v >= nv) return;
+    const ProjViewDev pv = views[i];
+    const AxisInfo ax = axis_info(pv.axis, g);
+    JRay r;
+    const size_t k = ((size_t)i * nv + v) * nu + u;
+    if (joseph_ray(pv, ax, u, v, r)) {
+        ab[k] = make_float4(r.a1, r.b1, r.a2, r.b2);
+        len[k] = r.L;
+    } else {
+        ab[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        len[k] = 0.f;
+    }
+}
+
+// Joseph transpose: one thread per voxel, gather over views and candidate
+// rays.  The candidates of a voxel are the pixels inside the projection of its
+// bilinear support on its main-axis slice (FP32 projective map, 0.05 px of
+// margin against rounding); each is then tested with the ray's own sample
+// position, exactly as the forward kernel computes it.
 __global__ void __launch_bounds__(256)
 joseph_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume,
-    const ProjViewDev* __restrict__ views, const double* __restrict__ c64, VolumeGeom g,
+    const ProjViewDev* __restrict__ views, const float* __restrict__ c32,
+    const float4* __restrict__ ray_ab, const float* __restrict__ ray_len, VolumeGeom g,
     int nu, int nv, int count, int accumulate)
 {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
@@ -189,59 +218,67 @@ joseph_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
     const int z = blockIdx.z;
     if (x >= g.nx || y >= g.ny) return;
     const int idx3[3] = { x, y, z };
-    const int dims[3] = { g.nx, g.ny, g.nz };
-    const double vox[3] = { g.vx, g.vy, g.vz };
-    const double org[3] = { g.ox, g.oy, g.oz };
+    const float vox[3] = { g.vx, g.vy, g.vz };
+    const float org[3] = { g.ox, g.oy, g.oz };
     const float margin = 0.05f;
     float acc = 0.f;
     for (int i = 0; i < count; ++i) {
         const ProjViewDev pv = views[i];
         const AxisInfo ax = axis_info(pv.axis, g);
         const int s = idx3[ax.p0], i1 = idx3[ax.p1], i2 = idx3[ax.p2];
-        const double* c = c64 + (size_t)i * 12;
-        // support of the voxel's bilinear weight on slice s: minor index in (i-1, i+1)
-        double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+        const float* c = c32 + (size_t)i * 12;
+        float umin = 1e30f, umax = -1e30f, vmin = 1e30f, vmax = -1e30f;
         bool ok = true;
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
-            double p[3];
-            p[ax.p0] = org[ax.p0] + double(s) * vox[ax.p0];
-            p[ax.p1] = org[ax.p1] + double(i1 + ((k & 1) ? 1 : -1)) * vox[ax.p1];
-            p[ax.p2] = org[ax.p2] + double(i2 + ((k & 2) ? 1 : -1)) * vox[ax.p2];
-            double uu, vv;
-            if (!project64(c, p[0], p[1], p[2], uu, vv)) { ok = false; break; }
-            umin = fmin(umin, uu); umax = fmax(umax, uu);
-            vmin = fmin(vmin, vv); vmax = fmax(vmax, vv);
+            float p[3];
+            p[ax.p0] = org[ax.p0] + float(s) * vox[ax.p0];
+            p[ax.p1] = org[ax.p1] + float(i1 + ((k & 1) ? 1 : -1)) * vox[ax.p1];
+            p[ax.p2] = org[ax.p2] + float(i2 + ((k & 2) ? 1 : -1)) * vox[ax.p2];
+            const float den = c[8] * p[0] + c[9] * p[1] + c[10] * p[2] + c[11];
+            if (!(den > 1e-6f)) { ok = false; break; }
+            const float rd = 1.f / den;
+            const float uu = (c[0] * p[0] + c[1] * p[1] + c[2] * p[2] + c[3]) * rd;
+            const float vv = (c[4] * p[0] + c[5] * p[1] + c[6] * p[2] + c[7]) * rd;
+            umin = fminf(umin, uu); umax = fmaxf(umax, uu);
+            vmin = fminf(vmin, vv); vmax = fmaxf(vmax, vv);
         }
         if (!ok) continue;
-        const int ulo = max(0, int(ceil(umin - margin)));
-        const int uhi = min(nu - 1, int(floor(umax + margin)));
-        const int vlo = max(0, int(ceil(vmin - margin)));
-        const int vhi = min(nv - 1, int(floor(vmax + margin)));
-        const float* pi = proj + (size_t)i * nu * nv;
+        const int ulo = max(0, int(ceilf(umin - margin)));
+        const int uhi = min(nu - 1, int(floorf(umax + margin)));
+        const int vlo = max(0, int(ceilf(vmin - margin)));
+        const int vhi = min(nv - 1, int(floorf(vmax + margin)));
+        const size_t vb = (size_t)i * nu * nv;
+        const float* pi = proj + vb;
+        const float4* abi = ray_ab + vb;
+        const float* li = ray_len + vb;
         for (int vv = vlo; vv <= vhi; ++vv)
             for (int uu = ulo; uu <= uhi; ++uu) {
+                const size_t k = (size_t)vv * nu + uu;
+                const float L = li[k];
+                if (L == 0.f) continue;
+                const float4 a = abi[k];
                 JRay r;
-                if (!joseph_ray(pv, ax, uu, vv, r)) continue;
+                r.a1 = a.x; r.b1 = a.y; r.a2 = a.z; r.b2 = a.w; r.L = L;
                 float f1, f2;
                 joseph_sample(r, ax, s, f1, f2);
                 const float g1 = floorf(f1), g2 = floorf(f2);
                 if (g1 < -1.f || g2 < -1.f || g1 > float(ax.n1 - 1) || g2 > float(ax.n2 - 1)) continue;
                 const int j1 = int(g1), j2 = int(g2);
                 const float w1 = f1 - g1, w2 = f2 - g2;
-                float a;
-                if (i1 == j1) a = 1.f - w1;
-                else if (i1 == j1 + 1) a = w1;
+                float wa;
+                if (i1 == j1) wa = 1.f - w1;
+                else if (i1 == j1 + 1) wa = w1;
                 else continue;
-                float b;
-                if (i2 == j2) b = 1.f - w2;
-                else if (i2 == j2 + 1) b = w2;
+                float wb;
+                if (i2 == j2) wb = 1.f - w2;
+                else if (i2 == j2 + 1) wb = w2;
                 else continue;
-                acc += pi[(size_t)vv * nu + uu] * r.L * (a * b);
+                acc += pi[k] * L * (wa * wb);
             }
     }
     float* dst = volume + ((size_t)z * g.ny + y) * g.nx + x;
     *dst = accumulate ? *dst + acc : acc;
-    (void)dims;
 }
 
 // ---------------------------------------------------------------------------
@@ -488,9 +525,11 @@ void launch_transpose_xy(const float* in, float* out, int nx, int ny, int nz, hi
 
 void launch_joseph_adjoint(const JosephAdjointLaunch& p, hipStream_t s)
 {
+    dim3 tb(64, 4), tg((p.nu + 63) / 64, (p.nv + 3) / 4, p.count);
+    hipLaunchKernelGGL(joseph_ray_table_kernel, tg, tb, 0, s, p.views, p.vol, p.nu, p.nv, p.ray_ab, p.ray_len);
     dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, p.vol.nz);
-    hipLaunchKernelGGL(joseph_adjoint_kernel, grid, block, 0, s, p.proj, p.volume, p.views, p.coeff64,
-        p.vol, p.nu, p.nv, p.count, p.accumulate);
+    hipLaunchKernelGGL(joseph_adjoint_kernel, grid, block, 0, s, p.proj, p.volume, p.views, p.coeff32,
+        p.ray_ab, p.ray_len, p.vol, p.nu, p.nv, p.count, p.accumulate);
     YK_HIP(hipGetLastError());
 }
 

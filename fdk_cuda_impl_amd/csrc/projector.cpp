@@ -91,6 +91,8 @@ yk_projector::yk_projector(yk_ctx* c, const yk_view* v, int n, int nu_, int nv_,
     d_jviews.upload(jv.data(), jv.size(), s);
     d_sviews.upload(sv.data(), sv.size(), s);
     d_c64.upload(c64.data(), c64.size(), s);
+    std::vector<float> c32(c64.begin(), c64.end());
+    d_c32.upload(c32.data(), c32.size(), s);
     d_partials.resize(1024);
     YK_HIP(hipStreamSynchronize(s));
 }
@@ -140,15 +142,25 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
     if (!d_volume || !d_proj) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
     hipStream_t s = ctx->stream;
     if (kind == YK_BP_JOSEPH_ADJOINT) {
-        JosephAdjointLaunch p{};
-        p.proj = d_proj;
-        p.volume = d_volume;
-        p.views = d_jviews.get() + first;
-        p.coeff64 = d_c64.get() + size_t(first) * 12;
-        p.vol = vol;
-        p.nu = nu; p.nv = nv; p.count = count;
-        p.accumulate = accumulate;
-        launch_joseph_adjoint(p, s);
+        // the ray table holds a chunk of views at a time (at most 1 GiB)
+        const size_t per_view = size_t(nu) * nv;
+        const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(count),
+            (size_t(1) << 30) / (per_view * (sizeof(float4) + sizeof(float))))));
+        d_ray_ab.resize(std::max(d_ray_ab.size(), per_view * chunk));
+        d_ray_len.resize(std::max(d_ray_len.size(), per_view * chunk));
+        for (int c0 = 0; c0 < count; c0 += chunk) {
+            JosephAdjointLaunch p{};
+            p.proj = d_proj + per_view * c0;
+            p.volume = d_volume;
+            p.views = d_jviews.get() + first + c0;
+            p.coeff32 = d_c32.get() + size_t(first + c0) * 12;
+            p.ray_ab = d_ray_ab.get();
+            p.ray_len = d_ray_len.get();
+            p.vol = vol;
+            p.nu = nu; p.nv = nv; p.count = std::min(chunk, count - c0);
+            p.accumulate = c0 > 0 ? 1 : accumulate;
+            launch_joseph_adjoint(p, s);
+        }
     } else if (kind == YK_BP_SIDDON_ADJOINT) {
         SiddonAdjointLaunch p{};
         p.proj = d_proj;

@@ -230,7 +230,7 @@ void launch_fdk_bp(const BpLaunch& p, hipStream_t s);
 // (packed half pair for half storage, float bits for bfloat16)
 void launch_row_factors(const float* scales, int n, int storage, int* gbits, unsigned* factors,
     hipStream_t s);
-int bp_block_z(int general);
+int bp_block_z(int general, int storage, int kahan);
 
 struct ProjViewDev {
     float src[4];               // voxel-normalized, volume-centred
@@ -264,7 +264,9 @@ struct JosephAdjointLaunch {
     const float* proj;          // [count][nv][nu]
     float* volume;              // [z][y][x]
     const ProjViewDev* views;   // device, `count`
-    const double* coeff64;      // device, per view cu[4] cv[4] cd[4] (world projective map)
+    const float* coeff32;       // device, per view cu[4] cv[4] cd[4] (world projective map)
+    float4* ray_ab;             // scratch [count][nv][nu]: a1, b1, a2, b2 of every ray
+    float* ray_len;             // scratch [count][nv][nu]: L (0: ray skipped)
     VolumeGeom vol;
     int nu, nv, count;
     int accumulate;
