@@ -40,23 +40,26 @@ __device__ __forceinline__ AxisInfo axis_info(int axis, const VolumeGeom& g)
     return a;
 }
 
-__device__ __forceinline__ float comp(const float* v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+template <typename T>
+__device__ __forceinline__ T pick(int k, T a, T b, T c) { return k == 0 ? a : (k == 1 ? b : c); }
 
 struct JRay {
     float a1, b1, a2, b2, L;
 };
 
-// Per-ray Joseph line in volume-centred voxel units.  Shared verbatim by the
-// forward kernel and the transpose, with contraction disabled so both
-// evaluate bit-identical parameters.
-__device__ __attribute__((noinline)) bool joseph_ray(const ProjViewDev& pv, const AxisInfo& ax,
+// Per-ray Joseph line in volume-centred voxel units.  Shared by the forward
+// kernel and the transpose's ray table, with contraction disabled so both
+// evaluate the same IEEE operations and obtain bit-identical parameters.
+__device__ __forceinline__ bool joseph_ray(const ProjViewDev& pv, const AxisInfo& ax,
     int u, int v, JRay& r)
 {
 #pragma clang fp contract(off)
-    float det[3];
-    for (int k = 0; k < 3; ++k) det[k] = (pv.detS[k] + float(u) * pv.detU[k]) + float(v) * pv.detV[k];
-    const float s0 = comp(pv.src, ax.p0), s1 = comp(pv.src, ax.p1), s2 = comp(pv.src, ax.p2);
-    const float d0 = comp(det, ax.p0), d1 = comp(det, ax.p1), d2 = comp(det, ax.p2);
+    const float dx = (pv.detS[0] + float(u) * pv.detU[0]) + float(v) * pv.detV[0];
+    const float dy = (pv.detS[1] + float(u) * pv.detU[1]) + float(v) * pv.detV[1];
+    const float dz = (pv.detS[2] + float(u) * pv.detU[2]) + float(v) * pv.detV[2];
+    const float sx = pv.src[0], sy = pv.src[1], sz = pv.src[2];
+    const float s0 = pick(ax.p0, sx, sy, sz), s1 = pick(ax.p1, sx, sy, sz), s2 = pick(ax.p2, sx, sy, sz);
+    const float d0 = pick(ax.p0, dx, dy, dz), d1 = pick(ax.p1, dx, dy, dz), d2 = pick(ax.p2, dx, dy, dz);
     const float den = s0 - d0;
     if (den == 0.f) return false;
     r.a1 = (s1 - d1) / den;
@@ -217,24 +220,27 @@ joseph_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     const int z = blockIdx.z;
     if (x >= g.nx || y >= g.ny) return;
-    const int idx3[3] = { x, y, z };
-    const float vox[3] = { g.vx, g.vy, g.vz };
-    const float org[3] = { g.ox, g.oy, g.oz };
     const float margin = 0.05f;
     float acc = 0.f;
     for (int i = 0; i < count; ++i) {
         const ProjViewDev pv = views[i];
         const AxisInfo ax = axis_info(pv.axis, g);
-        const int s = idx3[ax.p0], i1 = idx3[ax.p1], i2 = idx3[ax.p2];
+        const int s = pick(ax.p0, x, y, z), i1 = pick(ax.p1, x, y, z), i2 = pick(ax.p2, x, y, z);
         const float* c = c32 + (size_t)i * 12;
+        // world coordinates of the main-axis plane and of the minor neighbours
+        const float m0 = pick(ax.p0, g.ox, g.oy, g.oz) + float(s) * pick(ax.p0, g.vx, g.vy, g.vz);
+        const float o1 = pick(ax.p1, g.ox, g.oy, g.oz), w1 = pick(ax.p1, g.vx, g.vy, g.vz);
+        const float o2 = pick(ax.p2, g.ox, g.oy, g.oz), w2 = pick(ax.p2, g.vx, g.vy, g.vz);
         float umin = 1e30f, umax = -1e30f, vmin = 1e30f, vmax = -1e30f;
         bool ok = true;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
+            const float m1 = o1 + float(i1 + ((k & 1) ? 1 : -1)) * w1;
+            const float m2 = o2 + float(i2 + ((k & 2) ? 1 : -1)) * w2;
             float p[3];
-            p[ax.p0] = org[ax.p0] + float(s) * vox[ax.p0];
-            p[ax.p1] = org[ax.p1] + float(i1 + ((k & 1) ? 1 : -1)) * vox[ax.p1];
-            p[ax.p2] = org[ax.p2] + float(i2 + ((k & 2) ? 1 : -1)) * vox[ax.p2];
+            p[0] = ax.p0 == 0 ? m0 : (ax.p1 == 0 ? m1 : m2);
+            p[1] = ax.p0 == 1 ? m0 : (ax.p1 == 1 ? m1 : m2);
+            p[2] = ax.p0 == 2 ? m0 : (ax.p1 == 2 ? m1 : m2);
             const float den = c[8] * p[0] + c[9] * p[1] + c[10] * p[2] + c[11];
             if (!(den > 1e-6f)) { ok = false; break; }
             const float rd = 1.f / den;
