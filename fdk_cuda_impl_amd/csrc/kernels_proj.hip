@@ -105,6 +105,11 @@ __device__ __forceinline__ void joseph_slice_range(const JRay& r, const AxisInfo
     hi = min(ax.n0 - 1, int(ceilf(shi)) + 1);
 }
 
+// One ray per lane, 64 x 4 rays per workgroup.  The lanes of a wave march
+// the same slice index (the union of their slice ranges, lanes outside their
+// own range weigh zero), so a sample costs four unconditional loads at
+// clamped in-volume positions, with the off-volume taps zeroed in the weights;
+// four samples are issued before any is consumed.
 __global__ void __launch_bounds__(256)
 joseph_fp_kernel(const float* __restrict__ vol, const float* __restrict__ vol_t,
     float* __restrict__ proj, const ProjViewDev* __restrict__ views, VolumeGeom g,
@@ -113,7 +118,7 @@ joseph_fp_kernel(const float* __restrict__ vol, const float* __restrict__ vol_t,
     const int u = blockIdx.x * 64 + threadIdx.x;
     const int v = blockIdx.y * 4 + threadIdx.y;
     const int i = blockIdx.z;
-    if (u >= nu || v >= nv) return;
+    const bool pix = u < nu && v < nv;
     const ProjViewDev pv = views[i];
     const AxisInfo ax = axis_info(pv.axis, g);
     // memory strides of (s, i1, i2)
@@ -122,32 +127,49 @@ joseph_fp_kernel(const float* __restrict__ vol, const float* __restrict__ vol_t,
     if (pv.axis == 0) { base = vol_t; st0 = (size_t)g.ny; st2 = (size_t)g.nx * g.ny; }
     else if (pv.axis == 1) { base = vol; st0 = (size_t)g.nx; st2 = (size_t)g.nx * g.ny; }
     else { base = vol; st0 = (size_t)g.nx * g.ny; st2 = (size_t)g.nx; }
+    JRay r{};
+    int lo = 1, hi = 0;
+    if (pix && joseph_ray(pv, ax, u, v, r)) joseph_slice_range(r, ax, lo, hi);
+    // wave-wide slice range
+    int wlo = lo <= hi ? lo : 0x7fffffff, whi = lo <= hi ? hi : -1;
+    for (int off = 32; off > 0; off >>= 1) {
+        wlo = min(wlo, __shfl_xor(wlo, off, 64));
+        whi = max(whi, __shfl_xor(whi, off, 64));
+    }
+    const int n1 = ax.n1, n2 = ax.n2;
     float acc = 0.f;
-    JRay r;
-    float out = 0.f;
-    if (joseph_ray(pv, ax, u, v, r)) {
-        int lo, hi;
-        joseph_slice_range(r, ax, lo, hi);
-        for (int s = lo; s <= hi; ++s) {
+    constexpr int kUnroll = 4;
+    for (int s0 = wlo; s0 <= whi; s0 += kUnroll) {
+        float t[kUnroll][4], wa[kUnroll], wb[kUnroll], wc[kUnroll], wd[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+            const int s = min(s0 + k, whi);
             float f1, f2;
             joseph_sample(r, ax, s, f1, f2);
             const float g1 = floorf(f1), g2 = floorf(f2);
-            if (g1 < -1.f || g2 < -1.f || g1 > float(ax.n1 - 1) || g2 > float(ax.n2 - 1)) continue;
-            const int i1 = int(g1), i2 = int(g2);
             const float w1 = f1 - g1, w2 = f2 - g2;
+            const bool live = s0 + k <= whi && s >= lo && s <= hi && g1 >= -1.f && g2 >= -1.f &&
+                g1 <= float(n1 - 1) && g2 <= float(n2 - 1);
+            const int j1 = live ? int(g1) : 0, j2 = live ? int(g2) : 0;
+            // taps (j1 + a, j2 + b) inside the volume, the others weigh zero
+            const bool a0 = j1 >= 0, a1 = j1 + 1 < n1, b0 = j2 >= 0, b1 = j2 + 1 < n2;
+            const float x0 = live && a0 ? 1.f - w1 : 0.f, x1 = live && a1 ? w1 : 0.f;
+            const float y0 = b0 ? 1.f - w2 : 0.f, y1 = b1 ? w2 : 0.f;
+            wa[k] = x0 * y0; wb[k] = x1 * y0; wc[k] = x0 * y1; wd[k] = x1 * y1;
+            const int c1 = max(j1, 0), c1b = min(j1 + 1, n1 - 1);
+            const int c2 = max(j2, 0), c2b = min(j2 + 1, n2 - 1);
             const float* sl = base + (size_t)s * st0;
-            const bool ok1a = i1 >= 0, ok1b = i1 + 1 < ax.n1;
-            const bool ok2a = i2 >= 0, ok2b = i2 + 1 < ax.n2;
-            const float t00 = (ok1a && ok2a) ? sl[(size_t)i2 * st2 + i1] : 0.f;
-            const float t01 = (ok1b && ok2a) ? sl[(size_t)i2 * st2 + i1 + 1] : 0.f;
-            const float t10 = (ok1a && ok2b) ? sl[(size_t)(i2 + 1) * st2 + i1] : 0.f;
-            const float t11 = (ok1b && ok2b) ? sl[(size_t)(i2 + 1) * st2 + i1 + 1] : 0.f;
-            const float r0 = (1.f - w1) * t00 + w1 * t01;
-            const float r1 = (1.f - w1) * t10 + w1 * t11;
-            acc += (1.f - w2) * r0 + w2 * r1;
+            t[k][0] = sl[(size_t)c2 * st2 + c1];
+            t[k][1] = sl[(size_t)c2 * st2 + c1b];
+            t[k][2] = sl[(size_t)c2b * st2 + c1];
+            t[k][3] = sl[(size_t)c2b * st2 + c1b];
         }
-        out = acc * r.L;
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k)
+            acc += (wa[k] * t[k][0] + wb[k] * t[k][1]) + (wc[k] * t[k][2] + wd[k] * t[k][3]);
     }
+    if (!pix) return;
+    const float out = lo <= hi ? acc * r.L : 0.f;
     float* dst = proj + ((size_t)i * nv + v) * nu + u;
     *dst = accumulate ? *dst + out : out;
 }
