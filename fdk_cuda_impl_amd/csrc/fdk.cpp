@@ -120,11 +120,8 @@ yk_fdk::yk_fdk(yk_ctx* c, const yk_fdk_desc& d, const yk_view* v, int n, int nu_
     n_pad = padded_n(nu);
     filter_weights(desc.filter, n_pad, derived[0].du, weights);
     d_weights.upload(weights.data(), weights.size(), s);
-    std::vector<float2> tw(n_pad);
-    for (int k = 0; k < n_pad; ++k) {
-        const double a = -kTwoPi * double(k) / double(n_pad);
-        tw[k] = make_float2(float(std::cos(a)), float(std::sin(a)));
-    }
+    std::vector<float2> tw;
+    fft_twiddles(n_pad, tw);
     d_twiddles.upload(tw.data(), tw.size(), s);
 
     std::vector<FilterView> fv(n);

@@ -410,3 +410,22 @@ void projection_map(const yk_view& view, double c[12])
 }
 
 }  // namespace yk
+
+// Twiddles of the filter kernel's mixed radix-8/4/2 Stockham FFT, pass by pass:
+// for span ns and radix R the block holds exp(-2 pi i r k / (ns R)) at
+// [(r - 1) ns + k], r = 1..R-1, k = 0..ns-1.  n - 1 entries in all (n >= 2),
+// padded to n.
+void yk::fft_twiddles(int n, std::vector<float2>& out)
+{
+    out.clear();
+    for (int ns = 1; ns < n;) {
+        const int R = ns * 8 <= n ? 8 : (ns * 4 <= n ? 4 : 2);
+        for (int r = 1; r < R; ++r)
+            for (int k = 0; k < ns; ++k) {
+                const double a = -2.0 * 3.14159265358979323846 * double(r) * double(k) / double(ns * R);
+                out.push_back(make_float2(float(std::cos(a)), float(std::sin(a))));
+            }
+        ns *= R;
+    }
+    out.resize(size_t(std::max(n, 1)), make_float2(0.f, 0.f));
+}

@@ -138,6 +138,7 @@ void fdk_coefficients(const std::vector<yk_view>& views, const std::vector<Deriv
 void filter_weights(const yk_filter& f, int padded_n, double du, std::vector<float>& out);
 void validate_filter(const yk_filter& f);
 int padded_n(int nu);
+void fft_twiddles(int n, std::vector<float2>& out);
 int start_u(int nu, int n, double offset_u_pix);
 int main_axis(const yk_view& v, int nu, int nv);
 // world point -> detector pixel coordinates: u = (c[0..3].[X,1]) / (c[8..11].[X,1]),
