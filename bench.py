@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle accuracy/CPU leg")
+    ap.add_argument("--no-projectors", action="store_true",
+                    help="skip the forward-projector / transpose timings")
     ap.add_argument("--size", type=int, default=512, help="volume edge (default 512)")
     ap.add_argument("--views", type=int, default=720)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -71,6 +73,33 @@ def time_launches(fn, steps, warmup, stream):
     end.record(stream)
     torch.cuda.synchronize()
     return start.elapsed_time(end) / steps
+
+
+def projector_leg(yk, phantoms, ctx, stream):
+    """Forward projectors and their transposes at the iterative workload
+    (256^3 @ 0.5 mm, SOD 500 / SDD 1000, 360 views, 512 x 384 @ 0.6 mm,
+    random ellipsoids seed 4): milliseconds per call with HIP events on the
+    launch stream, and Joseph samples (rays x main-axis planes) per second."""
+    import torch
+    geom = yk.ScanGeometry(360, 360, 500.0, 1000.0, 512, 384, 0.6, 0.6, volume=(256, 256, 256),
+                           voxel_mm=(0.5, 0.5, 0.5))
+    grid = geom.grid()
+    ell = phantoms.random_ellipsoids(4, 32, semi_axes=(5.0, 30.0), density=(0.0, 0.05),
+                                     center_radius=25.0)
+    vol = ctx.phantom(ell, grid, clamp=False)
+    P = yk.Projector(ctx, geom.views(), geom.nu, geom.nv, grid)
+    proj = torch.empty((360, geom.nv, geom.nu), device="cuda")
+    back = torch.empty_like(vol)
+    out = {}
+    for kind in ("joseph", "siddon"):
+        out[f"{kind}_fp_ms"] = round(time_launches(lambda: P.forward(vol, kind=kind, out=proj), 3, 1, stream), 3)
+        out[f"{kind}_adjoint_ms"] = round(time_launches(lambda: P.adjoint(proj, kind=kind, out=back), 3, 1,
+                                                        stream), 3)
+    P.close()
+    samples = 360 * geom.nu * geom.nv * 256
+    out["joseph_fp_gsamples_per_s"] = round(samples / (out["joseph_fp_ms"] * 1e-3) / 1e9, 1)
+    out["workload"] = "256^3 @ 0.5 mm, 360 views, 512x384 @ 0.6 mm, SOD 500 / SDD 1000"
+    return out
 
 
 def oracle_check(views32, proj_cpu, geom, vols, args):
@@ -158,6 +187,10 @@ def main():
     print(f"[bench] fp32 backprojection {ms32:.2f} ms", file=sys.stderr, flush=True)
     ms16 = time_launches(lambda: plan16.backproject(f16, s16, volume=v16), args.steps, args.warmup, stream)
     print(f"[bench] fp16 backprojection {ms16:.2f} ms", file=sys.stderr, flush=True)
+    proj_leg = None
+    if not args.no_projectors:
+        proj_leg = projector_leg(yk, phantoms, ctx, stream)
+        print(f"[bench] projectors {proj_leg}", file=sys.stderr, flush=True)
     ctx.set_synchronous(True)
 
     nvox = s * s * s
@@ -185,6 +218,7 @@ def main():
         "fp16_speedup_vs_fp32": round(ms32 / ms16, 3),
         "filter_ms_fp16": round(filt_ms, 3),
         "joseph_fp_s": round(fp_s, 3),
+        "projectors": proj_leg,
     }
     if not args.no_cpu:
         print("[bench] FP64 oracle check", file=sys.stderr, flush=True)

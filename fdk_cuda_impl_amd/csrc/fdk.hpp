@@ -74,6 +74,10 @@ struct yk_projector {
     yk::DeviceArray<float> d_c32;      // the projective map in float (Joseph transpose candidates)
     yk::DeviceArray<float4> d_ray_ab;  // Joseph transpose ray table, one view chunk
     yk::DeviceArray<float> d_ray_len;
+    std::vector<char> separable;       // per view: main axis X/Y and detV along Z only
+    yk::DeviceArray<float2> d_sep_ab1; // separable Joseph transpose ray table, one view chunk
+    yk::DeviceArray<float2> d_sep_ab2;
+    yk::DeviceArray<float> d_sep_pl;
     yk::DeviceArray<float> d_vol_t;
     yk::DeviceArray<double> d_partials;
     yk::DeviceArray<yk::OpBpViewDev> d_opviews;  // operator-form FDK backprojection records

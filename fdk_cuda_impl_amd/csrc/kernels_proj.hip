@@ -71,13 +71,27 @@ __device__ __forceinline__ bool joseph_ray(const ProjViewDev& pv, const AxisInfo
     return true;
 }
 
-// minor-axis sample position of slice s (index space, texel k at k)
-__device__ __forceinline__ void joseph_sample(const JRay& r, const AxisInfo& ax, int s, float& f1, float& f2)
+// centred main-axis coordinate of slice s
+__device__ __forceinline__ float joseph_c0(int s, int n0)
 {
 #pragma clang fp contract(off)
-    const float c0 = float(s) - 0.5f * float(ax.n0 - 1);
-    f1 = (r.a1 * c0 + r.b1) + 0.5f * float(ax.n1 - 1);
-    f2 = (r.a2 * c0 + r.b2) + 0.5f * float(ax.n2 - 1);
+    return float(s) - 0.5f * float(n0 - 1);
+}
+
+// one minor-axis sample position (index space, texel k at k) of the line
+// c = a c0 + b
+__device__ __forceinline__ float joseph_minor(float a, float b, float c0, int n)
+{
+#pragma clang fp contract(off)
+    return (a * c0 + b) + 0.5f * float(n - 1);
+}
+
+// minor-axis sample position of slice s
+__device__ __forceinline__ void joseph_sample(const JRay& r, const AxisInfo& ax, int s, float& f1, float& f2)
+{
+    const float c0 = joseph_c0(s, ax.n0);
+    f1 = joseph_minor(r.a1, r.b1, c0, ax.n1);
+    f2 = joseph_minor(r.a2, r.b2, c0, ax.n2);
 }
 
 // slices whose sample can touch the volume: -1 < f < n on both minor axes
@@ -307,6 +321,183 @@ joseph_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
     }
     float* dst = volume + ((size_t)z * g.ny + y) * g.nx + x;
     *dst = accumulate ? *dst + acc : acc;
+}
+
+// ---------------------------------------------------------------------------
+// Joseph transpose for separable views: main axis X or Y and detector columns
+// parallel to Z (detV has no component on the main or first minor axis, as
+// for an untilted circular scan).  Then d0 and d1 of a ray do not depend on v,
+// so a1, b1 - and the first-minor sample f1 of every slice - are those of the
+// whole detector column u, while f2 is affine in v.
+// ---------------------------------------------------------------------------
+
+// Column-major ray table of the separable transpose: ab1[view][u] = (a1, b1)
+// of column u; ab2[view][u][v] = (a2, b2) and pl[view][u][v] = p * L of ray
+// (u, v) (0 for a ray the forward projector skips).  Same joseph_ray as the
+// forward kernel; the 32 x 32 tile is transposed through LDS so both the
+// projection reads and the table writes are contiguous.
+__global__ void __launch_bounds__(256)
+joseph_sep_table_kernel(const float* __restrict__ proj, const ProjViewDev* __restrict__ views,
+    VolumeGeom g, int nu, int nv, float2* __restrict__ ab1, float2* __restrict__ ab2,
+    float* __restrict__ pl)
+{
+    __shared__ float2 s_ab[32][33];
+    __shared__ float s_pl[32][33];
+    const int i = blockIdx.z;
+    const int u0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const ProjViewDev pv = views[i];
+    const AxisInfo ax = axis_info(pv.axis, g);
+    const float* pi = proj + (size_t)i * nu * nv;
+    for (int k = ty; k < 32; k += 8) {
+        const int u = u0 + tx, v = v0 + k;
+        float2 ab = make_float2(0.f, 0.f);
+        float w = 0.f;
+        if (u < nu && v < nv) {
+            JRay r;
+            const bool ok = joseph_ray(pv, ax, u, v, r);
+            if (ok) {
+                ab = make_float2(r.a2, r.b2);
+                w = pi[(size_t)v * nu + u] * r.L;
+            }
+            if (v == 0) ab1[(size_t)i * nu + u] = ok ? make_float2(r.a1, r.b1) : make_float2(0.f, 0.f);
+        }
+        s_ab[k][tx] = ab;
+        s_pl[k][tx] = w;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int u = u0 + k, v = v0 + tx;
+        if (u < nu && v < nv) {
+            const size_t o = ((size_t)i * nu + u) * nv + v;
+            ab2[o] = s_ab[tx][k];
+            pl[o] = s_pl[tx][k];
+        }
+    }
+}
+
+// world projective map (u or v numerator row r, denominator row d) of a point
+__device__ __forceinline__ float proj_coord(const float* c, int row, float x, float y, float z, float rd)
+{
+    return (c[row] * x + c[row + 1] * y + c[row + 2] * z + c[row + 3]) * rd;
+}
+
+// One lane per (x, y) column of kSepZ voxels, 16 x 16 columns per workgroup.
+// Per view the column sits on one main-axis slice s and one first-minor index
+// i1, so its candidate rays are whole detector columns u with
+// floor(f1(u)) in {i1 - 1, i1}, and in each such column the rays whose f2
+// lies in [z0 - 1, z0 + kSepZ).  Those are walked in v order; every ray adds
+// p L wa (1 - w2) and p L wa w2 to the two voxels its sample straddles.  The
+// sums live in LDS slots private to the lane (slot-major, lane-minor: no
+// bank conflicts, no sharing: plain read-modify-write, no atomics), so the
+// order of additions per voxel is fixed and the result deterministic.  Candidate ranges come from the FP32
+// projective map with 0.05 px of margin; membership is decided only by the
+// forward kernel's own f1, f2.
+constexpr int kSepZ = 32;
+constexpr int kSepU = 8;  // rays of a column loaded before any is consumed
+
+__global__ void __launch_bounds__(256)
+joseph_adjoint_sep_kernel(const float2* __restrict__ ab1, const float2* __restrict__ ab2,
+    const float* __restrict__ pl, const ProjViewDev* __restrict__ views, const float* __restrict__ c32,
+    VolumeGeom g, int nu, int nv, int count, int accumulate, float* __restrict__ volume)
+{
+    __shared__ float slot[(kSepZ + 2) * 256];
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x * 16 + (lane & 15);
+    const int y = blockIdx.y * 16 + (lane >> 4);
+    const int z0 = blockIdx.z * kSepZ;
+#pragma unroll
+    for (int k = 0; k < kSepZ + 2; ++k) slot[k * 256 + lane] = 0.f;
+    if (x >= g.nx || y >= g.ny) return;
+    const float margin = 0.05f;
+    const float zlo = g.oz + float(z0 - 1) * g.vz;
+    const float zhi = g.oz + float(z0 + kSepZ) * g.vz;
+    const float zmid = g.oz + (float(z0) + 0.5f * float(kSepZ - 1)) * g.vz;
+    for (int i = 0; i < count; ++i) {
+        const ProjViewDev pv = views[i];
+        const AxisInfo ax = axis_info(pv.axis, g);
+        const bool xmain = pv.axis == 0;
+        const int s = xmain ? x : y, i1 = xmain ? y : x;
+        const float c0 = joseph_c0(s, ax.n0);
+        const float* c = c32 + (size_t)i * 12;
+        const float m0 = (xmain ? g.ox : g.oy) + float(s) * ax.v0;
+        const float o1 = xmain ? g.oy : g.ox;
+        // detector columns whose rays cross slice s within (i1 - 1, i1 + 1)
+        float ua, ub;
+        {
+            const float ma = o1 + float(i1 - 1) * ax.v1, mb = o1 + float(i1 + 1) * ax.v1;
+            const float xa = xmain ? m0 : ma, ya = xmain ? ma : m0;
+            const float xb = xmain ? m0 : mb, yb = xmain ? mb : m0;
+            const float da = c[8] * xa + c[9] * ya + c[10] * zmid + c[11];
+            const float db = c[8] * xb + c[9] * yb + c[10] * zmid + c[11];
+            if (!(da > 1e-6f) || !(db > 1e-6f)) continue;
+            ua = proj_coord(c, 0, xa, ya, zmid, 1.f / da);
+            ub = proj_coord(c, 0, xb, yb, zmid, 1.f / db);
+        }
+        const int ulo = max(0, int(ceilf(fminf(ua, ub) - margin)));
+        const int uhi = min(nu - 1, int(floorf(fmaxf(ua, ub) + margin)));
+        for (int u = ulo; u <= uhi; ++u) {
+            const float2 q1 = ab1[(size_t)i * nu + u];
+            const float f1 = joseph_minor(q1.x, q1.y, c0, ax.n1);
+            const float g1 = floorf(f1);
+            if (g1 < -1.f || g1 > float(ax.n1 - 1)) continue;
+            const int j1 = int(g1);
+            const float w1 = f1 - g1;
+            float wa;
+            if (j1 == i1) wa = 1.f - w1;
+            else if (j1 + 1 == i1) wa = w1;
+            else continue;
+            // rays of column u whose f2 falls in [z0 - 1, z0 + kSepZ)
+            const float m1 = o1 + f1 * ax.v1;
+            const float xc = xmain ? m0 : m1, yc = xmain ? m1 : m0;
+            const float dl = c[8] * xc + c[9] * yc + c[10] * zlo + c[11];
+            const float dh = c[8] * xc + c[9] * yc + c[10] * zhi + c[11];
+            if (!(dl > 1e-6f) || !(dh > 1e-6f)) continue;
+            const float va = proj_coord(c, 4, xc, yc, zlo, 1.f / dl);
+            const float vb = proj_coord(c, 4, xc, yc, zhi, 1.f / dh);
+            const int vlo = max(0, int(ceilf(fminf(va, vb) - margin)));
+            const int vhi = min(nv - 1, int(floorf(fmaxf(va, vb) + margin)));
+            const size_t col = ((size_t)i * nu + u) * nv;
+            const float2* q2 = ab2 + col;
+            const float* w = pl + col;
+            // kSepU rays per step, all loads issued first (clamped to the
+            // last ray; the excess and the rays outside the column add zero)
+            const int nr = vhi - vlo + 1;
+            for (int r0 = 0; r0 < nr; r0 += kSepU) {
+                float2 q[kSepU];
+                float pw[kSepU];
+#pragma unroll
+                for (int j = 0; j < kSepU; ++j) {
+                    const int v = vlo + min(r0 + j, nr - 1);
+                    q[j] = q2[v];
+                    pw[j] = w[v];
+                }
+#pragma unroll
+                for (int j = 0; j < kSepU; ++j) {
+                    const float f2 = joseph_minor(q[j].x, q[j].y, c0, ax.n2);
+                    const float g2 = floorf(f2);
+                    const int k = int(g2) - z0 + 1;  // slot of voxel floor(f2)
+                    const bool live = r0 + j < nr && g2 >= -1.f && g2 <= float(ax.n2 - 1) && k >= 0 &&
+                        k <= kSepZ;
+                    const float w2 = f2 - g2;
+                    const float cw = live ? pw[j] * wa : 0.f;
+                    float* sk = slot + ((live ? k : 0) * 256 + lane);
+                    sk[0] += cw * (1.f - w2);
+                    sk[256] += cw * w2;
+                }
+            }
+        }
+    }
+    const size_t plane = (size_t)g.nx * g.ny;
+    const size_t colx = (size_t)y * g.nx + x;
+#pragma unroll
+    for (int k = 1; k <= kSepZ; ++k) {
+        const int z = z0 + k - 1;
+        if (z >= g.nz) break;
+        const float val = slot[k * 256 + lane];
+        float* dst = volume + z * plane + colx;
+        *dst = accumulate ? *dst + val : val;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -558,6 +749,17 @@ void launch_joseph_adjoint(const JosephAdjointLaunch& p, hipStream_t s)
     dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, p.vol.nz);
     hipLaunchKernelGGL(joseph_adjoint_kernel, grid, block, 0, s, p.proj, p.volume, p.views, p.coeff32,
         p.ray_ab, p.ray_len, p.vol, p.nu, p.nv, p.count, p.accumulate);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_joseph_adjoint_sep(const JosephSepLaunch& p, hipStream_t s)
+{
+    dim3 tb(256), tg((p.nu + 31) / 32, (p.nv + 31) / 32, p.count);
+    hipLaunchKernelGGL(joseph_sep_table_kernel, tg, tb, 0, s, p.proj, p.views, p.vol, p.nu, p.nv, p.ab1,
+        p.ab2, p.pl);
+    dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, (p.vol.nz + kSepZ - 1) / kSepZ);
+    hipLaunchKernelGGL(joseph_adjoint_sep_kernel, grid, block, 0, s, p.ab1, p.ab2, p.pl, p.views, p.coeff32,
+        p.vol, p.nu, p.nv, p.count, p.accumulate, p.volume);
     YK_HIP(hipGetLastError());
 }
 

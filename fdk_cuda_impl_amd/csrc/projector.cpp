@@ -42,6 +42,9 @@ yk_projector::yk_projector(yk_ctx* c, const yk_view* v, int n, int nu_, int nv_,
         }
         p.axis = main_axis(w, nu, nv);
         if (p.axis == 0) any_x_axis = true;
+        // d0 and d1 of every ray independent of v (exact zeros, so bit-exact)
+        separable.push_back(p.axis != 2 && p.detV[p.axis == 0 ? 0 : 1] == 0.f &&
+                            p.detV[p.axis == 0 ? 1 : 0] == 0.f);
         SiddonViewDev& s = sv[i];
         std::memset(&s, 0, sizeof(s));
         std::memcpy(s.src, w.src, sizeof(s.src));
@@ -141,7 +144,31 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
     if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
     if (!d_volume || !d_proj) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
     hipStream_t s = ctx->stream;
-    if (kind == YK_BP_JOSEPH_ADJOINT) {
+    bool all_sep = true;
+    for (int i = first; i < first + count; ++i) all_sep = all_sep && separable[i];
+    if (kind == YK_BP_JOSEPH_ADJOINT && all_sep) {
+        // column-major ray table of a chunk of views at a time (at most 1 GiB)
+        const size_t per_view = size_t(nu) * nv;
+        const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(count),
+            (size_t(1) << 30) / (per_view * (sizeof(float2) + sizeof(float)) + size_t(nu) * sizeof(float2)))));
+        d_sep_ab1.resize(std::max(d_sep_ab1.size(), size_t(nu) * chunk));
+        d_sep_ab2.resize(std::max(d_sep_ab2.size(), per_view * chunk));
+        d_sep_pl.resize(std::max(d_sep_pl.size(), per_view * chunk));
+        for (int c0 = 0; c0 < count; c0 += chunk) {
+            JosephSepLaunch p{};
+            p.proj = d_proj + per_view * c0;
+            p.volume = d_volume;
+            p.views = d_jviews.get() + first + c0;
+            p.coeff32 = d_c32.get() + size_t(first + c0) * 12;
+            p.ab1 = d_sep_ab1.get();
+            p.ab2 = d_sep_ab2.get();
+            p.pl = d_sep_pl.get();
+            p.vol = vol;
+            p.nu = nu; p.nv = nv; p.count = std::min(chunk, count - c0);
+            p.accumulate = c0 > 0 ? 1 : accumulate;
+            launch_joseph_adjoint_sep(p, s);
+        }
+    } else if (kind == YK_BP_JOSEPH_ADJOINT) {
         // the ray table holds a chunk of views at a time (at most 1 GiB)
         const size_t per_view = size_t(nu) * nv;
         const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(count),

@@ -274,6 +274,22 @@ struct JosephAdjointLaunch {
 };
 void launch_joseph_adjoint(const JosephAdjointLaunch& p, hipStream_t s);
 
+// Transpose of the Joseph projector for separable views (main axis X or Y,
+// detV with no X/Y component); the ray table arrays are scratch of `count` views.
+struct JosephSepLaunch {
+    const float* proj;          // [count][nv][nu]
+    float* volume;              // [z][y][x]
+    const ProjViewDev* views;   // device, `count`
+    const float* coeff32;       // device, per view cu[4] cv[4] cd[4] (world projective map)
+    float2* ab1;                // scratch [count][nu]
+    float2* ab2;                // scratch [count][nu][nv]
+    float* pl;                  // scratch [count][nu][nv]
+    VolumeGeom vol;
+    int nu, nv, count;
+    int accumulate;
+};
+void launch_joseph_adjoint_sep(const JosephSepLaunch& p, hipStream_t s);
+
 struct SiddonLaunch {
     const float* volume;
     float* proj;

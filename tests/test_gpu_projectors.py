@@ -15,6 +15,8 @@ GEOMS = {
     "plain": dict(),
     "offset": dict(offset_unv_mm=(3.0, 0.0, -2.0), center_mm=(1.5, -1.0, 0.5), start_angle_rad=0.2),
     "aniso": dict(volume=(30, 22, 17), voxel_mm=(0.9, 1.1, 1.3), tilt_rad=(0.02, 0.01, -0.03)),
+    # several 32-slice blocks of the separable (untilted) Joseph transpose
+    "tall": dict(volume=(20, 16, 70), voxel_mm=(1.0, 1.0, 0.5), rows=80),
 }
 
 
@@ -54,9 +56,12 @@ def test_adjoint_dot_product(ctx, name, kind):
     assert abs(lhs - rhs) / abs(lhs) < 1e-5, (lhs, rhs)
 
 
+@pytest.mark.parametrize("name", ["offset", "aniso", "tall"])
 @pytest.mark.parametrize("kind", ["joseph", "siddon"])
-def test_adjoint_matches_oracle_transpose(ctx, oracle, kind):
-    g = small_geometry(**GEOMS["offset"])
+def test_adjoint_matches_oracle_transpose(ctx, oracle, kind, name):
+    """Untilted scans take the separable Joseph transpose (column walk), the
+    tilted one the per-voxel candidate search; both against the FP64 oracle."""
+    g = small_geometry(**GEOMS[name])
     P = yk.Projector(ctx, g.views(), g.nu, g.nv, g.grid())
     rng = np.random.default_rng(13)
     y = rng.standard_normal((g.total_views, g.nv, g.nu)).astype(np.float32)
@@ -64,7 +69,9 @@ def test_adjoint_matches_oracle_transpose(ctx, oracle, kind):
     ov = oracle.views_from_float(P.views)
     fn = oracle.bp_joseph_adjoint if kind == "joseph" else oracle.bp_siddon_adjoint
     ref = fn(ov, g.nu, g.nv, oracle_volume(oracle, g), y.astype(np.float64))
-    assert rel_rmse(got, ref) < 1e-5, rel_rmse(got, ref)
+    # zero-mean input: the FP32 sums cancel, so the relative error is a few
+    # FP32 ulps of the summands rather than of the result
+    assert rel_rmse(got, ref) < 2e-5, rel_rmse(got, ref)
 
 
 @pytest.mark.parametrize("kind", ["fdk", "fdk_pseudo_matched"])
