@@ -333,12 +333,13 @@ joseph_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
 
 // Column-major ray table of the separable transpose: ab1[view][u] = (a1, b1)
 // of column u; ab2[view][u][v] = (a2, b2) and pl[view][u][v] = p * L of ray
-// (u, v) (0 for a ray the forward projector skips).  Same joseph_ray as the
+// (u, v) (0 for a ray the forward projector skips), each column padded to
+// nvp = nv + kSepPad entries with zeros.  Same joseph_ray as the
 // forward kernel; the 32 x 32 tile is transposed through LDS so both the
 // projection reads and the table writes are contiguous.
 __global__ void __launch_bounds__(256)
 joseph_sep_table_kernel(const float* __restrict__ proj, const ProjViewDev* __restrict__ views,
-    VolumeGeom g, int nu, int nv, float2* __restrict__ ab1, float2* __restrict__ ab2,
+    VolumeGeom g, int nu, int nv, int nvp, float2* __restrict__ ab1, float2* __restrict__ ab2,
     float* __restrict__ pl)
 {
     __shared__ float2 s_ab[32][33];
@@ -368,8 +369,8 @@ joseph_sep_table_kernel(const float* __restrict__ proj, const ProjViewDev* __res
     __syncthreads();
     for (int k = ty; k < 32; k += 8) {
         const int u = u0 + k, v = v0 + tx;
-        if (u < nu && v < nv) {
-            const size_t o = ((size_t)i * nu + u) * nv + v;
+        if (u < nu && v < nvp) {
+            const size_t o = ((size_t)i * nu + u) * nvp + v;
             ab2[o] = s_ab[tx][k];
             pl[o] = s_pl[tx][k];
         }
@@ -394,12 +395,12 @@ __device__ __forceinline__ float proj_coord(const float* c, int row, float x, fl
 // projective map with 0.05 px of margin; membership is decided only by the
 // forward kernel's own f1, f2.
 constexpr int kSepZ = 32;
-constexpr int kSepU = 8;  // rays of a column loaded before any is consumed
+constexpr int kSepU = kSepPad;  // rays of a column loaded before any is consumed
 
 __global__ void __launch_bounds__(256)
 joseph_adjoint_sep_kernel(const float2* __restrict__ ab1, const float2* __restrict__ ab2,
     const float* __restrict__ pl, const ProjViewDev* __restrict__ views, const float* __restrict__ c32,
-    VolumeGeom g, int nu, int nv, int count, int accumulate, float* __restrict__ volume)
+    VolumeGeom g, int nu, int nv, int nvp, int count, int accumulate, float* __restrict__ volume)
 {
     __shared__ float slot[(kSepZ + 2) * 256];
     const int lane = threadIdx.x;
@@ -457,28 +458,30 @@ joseph_adjoint_sep_kernel(const float2* __restrict__ ab1, const float2* __restri
             const float vb = proj_coord(c, 4, xc, yc, zhi, 1.f / dh);
             const int vlo = max(0, int(ceilf(fminf(va, vb) - margin)));
             const int vhi = min(nv - 1, int(floorf(fmaxf(va, vb) + margin)));
-            const size_t col = ((size_t)i * nu + u) * nv;
-            const float2* q2 = ab2 + col;
-            const float* w = pl + col;
-            // kSepU rays per step, all loads issued first (clamped to the
-            // last ray; the excess and the rays outside the column add zero)
+            // kSepU rays per step, loads first; the column carries kSepU
+            // zero-weight pad entries, so a step may run past vhi (rays
+            // beyond the candidate range fall outside the slots by f2)
+            const float2* q2 = ab2 + ((size_t)i * nu + u) * nvp + vlo;
+            const float* w = pl + ((size_t)i * nu + u) * nvp + vlo;
             const int nr = vhi - vlo + 1;
             for (int r0 = 0; r0 < nr; r0 += kSepU) {
                 float2 q[kSepU];
                 float pw[kSepU];
 #pragma unroll
                 for (int j = 0; j < kSepU; ++j) {
-                    const int v = vlo + min(r0 + j, nr - 1);
-                    q[j] = q2[v];
-                    pw[j] = w[v];
+                    q[j] = q2[r0 + j];
+                    pw[j] = w[r0 + j];
                 }
 #pragma unroll
                 for (int j = 0; j < kSepU; ++j) {
                     const float f2 = joseph_minor(q[j].x, q[j].y, c0, ax.n2);
                     const float g2 = floorf(f2);
-                    const int k = int(g2) - z0 + 1;  // slot of voxel floor(f2)
-                    const bool live = r0 + j < nr && g2 >= -1.f && g2 <= float(ax.n2 - 1) && k >= 0 &&
-                        k <= kSepZ;
+                    // slot of voxel floor(f2): 0 .. kSepZ covers f2 in
+                    // [z0 - 1, z0 + kSepZ); slots of voxels outside the
+                    // volume are never stored, which stands in for the
+                    // forward kernel's g2 range test
+                    const int k = int(g2) - (z0 - 1);
+                    const bool live = unsigned(k) <= unsigned(kSepZ);
                     const float w2 = f2 - g2;
                     const float cw = live ? pw[j] * wa : 0.f;
                     float* sk = slot + ((live ? k : 0) * 256 + lane);
@@ -754,12 +757,13 @@ void launch_joseph_adjoint(const JosephAdjointLaunch& p, hipStream_t s)
 
 void launch_joseph_adjoint_sep(const JosephSepLaunch& p, hipStream_t s)
 {
-    dim3 tb(256), tg((p.nu + 31) / 32, (p.nv + 31) / 32, p.count);
-    hipLaunchKernelGGL(joseph_sep_table_kernel, tg, tb, 0, s, p.proj, p.views, p.vol, p.nu, p.nv, p.ab1,
+    const int nvp = p.nv + kSepPad;
+    dim3 tb(256), tg((p.nu + 31) / 32, (nvp + 31) / 32, p.count);
+    hipLaunchKernelGGL(joseph_sep_table_kernel, tg, tb, 0, s, p.proj, p.views, p.vol, p.nu, p.nv, nvp, p.ab1,
         p.ab2, p.pl);
     dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, (p.vol.nz + kSepZ - 1) / kSepZ);
     hipLaunchKernelGGL(joseph_adjoint_sep_kernel, grid, block, 0, s, p.ab1, p.ab2, p.pl, p.views, p.coeff32,
-        p.vol, p.nu, p.nv, p.count, p.accumulate, p.volume);
+        p.vol, p.nu, p.nv, nvp, p.count, p.accumulate, p.volume);
     YK_HIP(hipGetLastError());
 }
 

@@ -149,11 +149,12 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
     if (kind == YK_BP_JOSEPH_ADJOINT && all_sep) {
         // column-major ray table of a chunk of views at a time (at most 1 GiB)
         const size_t per_view = size_t(nu) * nv;
+        const size_t table = size_t(nu) * (nv + kSepPad);
         const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(count),
-            (size_t(1) << 30) / (per_view * (sizeof(float2) + sizeof(float)) + size_t(nu) * sizeof(float2)))));
+            (size_t(1) << 30) / (table * (sizeof(float2) + sizeof(float)) + size_t(nu) * sizeof(float2)))));
         d_sep_ab1.resize(std::max(d_sep_ab1.size(), size_t(nu) * chunk));
-        d_sep_ab2.resize(std::max(d_sep_ab2.size(), per_view * chunk));
-        d_sep_pl.resize(std::max(d_sep_pl.size(), per_view * chunk));
+        d_sep_ab2.resize(std::max(d_sep_ab2.size(), table * chunk));
+        d_sep_pl.resize(std::max(d_sep_pl.size(), table * chunk));
         for (int c0 = 0; c0 < count; c0 += chunk) {
             JosephSepLaunch p{};
             p.proj = d_proj + per_view * c0;

@@ -274,6 +274,9 @@ struct JosephAdjointLaunch {
 };
 void launch_joseph_adjoint(const JosephAdjointLaunch& p, hipStream_t s);
 
+// zero entries closing each detector column of the separable ray table
+constexpr int kSepPad = 8;
+
 // Transpose of the Joseph projector for separable views (main axis X or Y,
 // detV with no X/Y component); the ray table arrays are scratch of `count` views.
 struct JosephSepLaunch {
@@ -282,8 +285,8 @@ struct JosephSepLaunch {
     const ProjViewDev* views;   // device, `count`
     const float* coeff32;       // device, per view cu[4] cv[4] cd[4] (world projective map)
     float2* ab1;                // scratch [count][nu]
-    float2* ab2;                // scratch [count][nu][nv]
-    float* pl;                  // scratch [count][nu][nv]
+    float2* ab2;                // scratch [count][nu][nv + kSepPad]
+    float* pl;                  // scratch [count][nu][nv + kSepPad]
     VolumeGeom vol;
     int nu, nv, count;
     int accumulate;
