@@ -88,6 +88,43 @@ def operator_norm_sq(projector: Projector, kind: str = "joseph", iterations: int
     return lam
 
 
+def cgls(projector: Projector, b: torch.Tensor, iterations: int = 10, kind: str = "joseph", x0=None):
+    """CGLS on ||A x - b||^2 with A and A^T applied through the differentiable
+    operators above (the CGLS recursion of CglsRobustBackend / AstraClassic,
+    src/Reconstruction/Iterative/Flat/YkCglsBackends.hpp:22-190; dots in FP64
+    on the device, without FP64 copies of the vectors).  Returns x and
+    ||b - A x|| before and after every iteration."""
+    dot = projector.ctx.dot
+    with torch.no_grad():
+        if x0 is None:
+            x = torch.zeros(projector.volume_shape, device=b.device)
+            r = b.clone()
+        else:
+            x = x0.detach().clone()
+            r = b - project(x, projector, kind)
+        s = backproject(r, projector, kind)
+        p = s.clone()
+        gamma = dot(s, s)
+        history = [math.sqrt(dot(r, r))]
+        for _ in range(iterations):
+            if gamma == 0.0:
+                break
+            q = project(p, projector, kind)
+            qq = dot(q, q)
+            if qq == 0.0:
+                break
+            alpha = gamma / qq
+            x.add_(p, alpha=alpha)
+            r.sub_(q, alpha=alpha)
+            del q
+            s = backproject(r, projector, kind)
+            gamma_new = dot(s, s)
+            p.mul_(gamma_new / gamma).add_(s)
+            gamma = gamma_new
+            history.append(math.sqrt(dot(r, r)))
+    return x, history
+
+
 def least_squares_tv(projector: Projector, b: torch.Tensor, context: Context, x0=None, steps: int = 50,
                      lam: float = 1e-3, eps: float = 1e-6, kind: str = "joseph", nonnegative: bool = False):
     """Gradient descent on f(x) = 0.5 ||A x - b||^2 + lam TV_eps(x).
@@ -104,7 +141,7 @@ def least_squares_tv(projector: Projector, b: torch.Tensor, context: Context, x0
         r = project(v, projector, kind) - b
         f = 0.5 * torch.sum(r.double() * r.double()) + lam * smoothed_tv(v, context, grid, eps)
         (g,) = torch.autograd.grad(f, v)
-        return float(f), g
+        return float(f.detach()), g
 
     def objective(v):
         r = projector.forward(v, kind=kind) - b

@@ -1,0 +1,175 @@
+"""The iterative and the largest reference configurations at full size on one GPU.
+
+* 256^3 @ 0.5 mm, SOD 500 / SDD 1000, 360 views, 512 x 384 @ 0.6 mm; random
+  ellipsoids (seed 4); counts ~ Poisson(1e5 exp(-p)) (numpy seed 4),
+  b = -ln(max(c, 1) / 1e5).  30 CGLS iterations, then 50 gradient steps on
+  0.5 ||A x - b||^2 + 1e-3 TV_eps (eps = 1e-6); the gradient of that
+  objective against central differences on 64 random voxels.
+* 1024^3 @ 0.15 mm, SOD 700 / SDD 1200, 1440 views, 1536^2 @ 0.2 mm; 256
+  random ellipsoids (seed 5); FP32 and FP16 + Kahan FDK backprojection (a
+  16-view slice of the scan against the FP64 oracle on 4096 random voxels,
+  the whole scan FP16 against FP32), and 10 CGLS iterations through the
+  differentiable operators.
+"""
+import numpy as np
+import pytest
+import torch
+
+import fdk_cuda_impl_amd as yk
+from fdk_cuda_impl_amd import autograd as ag
+from fdk_cuda_impl_amd import phantoms
+from helpers import rel_rmse
+
+pytestmark = pytest.mark.gpu
+
+
+def c4_geometry():
+    return yk.ScanGeometry(360, 360, 500.0, 1000.0, 512, 384, 0.6, 0.6, volume=(256, 256, 256),
+                           voxel_mm=(0.5, 0.5, 0.5))
+
+
+@pytest.fixture(scope="module")
+def c4(ctx):
+    g = c4_geometry()
+    P = yk.Projector(ctx, g.views(), g.nu, g.nv, g.grid())
+    ell = phantoms.random_ellipsoids(4, 64, semi_axes=(5.0, 30.0), density=(0.0, 0.05), center_radius=30.0)
+    truth = ctx.phantom(ell, g.grid(), clamp=False)
+    p = P.forward(truth).cpu().numpy().astype(np.float64)
+    rng = np.random.default_rng(4)
+    counts = rng.poisson(1e5 * np.exp(-p))
+    b = torch.from_numpy((-np.log(np.maximum(counts, 1) / 1e5)).astype(np.float32)).cuda()
+    x, res = P.cgls(b, iterations=30, forward="joseph", back="joseph")
+    yield g, P, truth, b, x, res
+    P.close()
+
+
+def test_c4_cgls_30_iterations(c4):
+    g, P, truth, b, x, res = c4
+    print(f"CGLS 30 at 256^3 x 360: ||r|| {res[0]:.4e} -> {res[-1]:.4e}, "
+          f"rel error vs phantom {rel_rmse(x.cpu().numpy(), truth.cpu().numpy()):.3f}")
+    assert np.all(np.diff(res) <= 1e-6 * res[0])
+    assert res[-1] < 0.05 * res[0]
+    assert rel_rmse(x.cpu().numpy(), truth.cpu().numpy()) < 0.5
+
+
+def test_c4_tv_gradient_steps_and_finite_differences(ctx, oracle, c4):
+    g, P, truth, b, x_cgls, _ = c4
+    lam, eps = 1e-3, 1e-6
+    x, hist = ag.least_squares_tv(P, b, ctx, x0=x_cgls, steps=50, lam=lam, eps=eps)
+    print(f"50 TV-regularised steps: objective {hist[0]:.6e} -> {hist[-1]:.6e}")
+    assert len(hist) == 51
+    assert all(h1 <= h0 * (1 + 1e-9) for h0, h1 in zip(hist, hist[1:]))
+    assert hist[-1] < hist[0]
+
+    # gradient of the objective at the final iterate
+    xt = x.detach().clone().requires_grad_(True)
+    r = ag.project(xt, P) - b
+    f = 0.5 * torch.sum(r.double() * r.double()) + lam * ag.smoothed_tv(xt, ctx, g.grid(), eps)
+    f.backward()
+    grad = xt.grad.cpu().numpy()
+    xh = x.detach().cpu().numpy().astype(np.float64)
+    nz, ny, nx = xh.shape
+    rng = np.random.default_rng(3)
+    picks = np.stack([rng.integers(2, n - 2, 64) for n in (nz, ny, nx)], axis=1)
+    vol5 = oracle.volume(5, 5, 5, 0.5, 0.5, 0.5)
+    h_data, h_tv = 0.25, 1e-5
+    worst = 0.0
+    with torch.no_grad():
+        base = x.detach().clone()
+        for (k, j, i) in picks:
+            # data term: quadratic, so a central difference is exact
+            xp, xm = base.clone(), base.clone()
+            xp[k, j, i] += h_data
+            xm[k, j, i] -= h_data
+            rp = (P.forward(xp) - b).double()
+            rm = (P.forward(xm) - b).double()
+            fd = 0.5 * float(torch.sum(rp * rp) - torch.sum(rm * rm)) / (2 * h_data)
+            # TV term: every difference that involves voxel (k, j, i) lies in
+            # the 5^3 patch around it, so the patch's TV changes exactly as
+            # the volume's does
+            patch = xh[k - 2:k + 3, j - 2:j + 3, i - 2:i + 3].copy()
+            pp, pm = patch.copy(), patch.copy()
+            pp[2, 2, 2] += h_tv
+            pm[2, 2, 2] -= h_tv
+            tp, _ = oracle.tv_value_grad(pp, vol5, eps)
+            tm, _ = oracle.tv_value_grad(pm, vol5, eps)
+            fd += lam * (tp - tm) / (2 * h_tv)
+            worst = max(worst, abs(fd - grad[k, j, i]))
+    print(f"finite differences on 64 voxels: worst |fd - grad| {worst:.3e} (max |grad| {np.abs(grad).max():.3e})")
+    assert worst < 1e-3 * np.abs(grad).max()
+
+
+def c5_geometry():
+    return yk.ScanGeometry(1440, 1440, 700.0, 1200.0, 1536, 1536, 0.2, 0.2, volume=(1024, 1024, 1024),
+                           voxel_mm=(0.15, 0.15, 0.15))
+
+
+@pytest.fixture(scope="module")
+def c5(ctx):
+    g = c5_geometry()
+    P = yk.Projector(ctx, g.views(), g.nu, g.nv, g.grid())
+    ell = phantoms.random_ellipsoids(5, 256, semi_axes=(5.0, 40.0), density=(0.0, 0.05), center_radius=50.0)
+    truth = ctx.phantom(ell, g.grid(), clamp=False)
+    proj = P.forward(truth)
+    yield g, P, truth, proj
+    P.close()
+
+
+def timed_backprojection(plan, f, s, vol):
+    plan.backproject(f, s, volume=vol)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    plan.backproject(f, s, volume=vol)
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end)
+
+
+def test_c5_fdk_fp32_and_fp16(ctx, oracle, c5):
+    g, P, _, proj = c5
+    first, count = 704, 16
+    sub = proj[first:first + count].cpu().numpy().astype(np.float64)
+    # FP64 oracle of the 16-view slice: derived geometry of the whole scan
+    ov = oracle.views_from_float(g.views())
+    d = oracle.derive(ov, g.nu, g.nv, 2 * np.pi)
+    c = oracle.coeffs(ov, d)
+    q = oracle.preweight(ov[first:first + count], d[first:first + count], sub)
+    n = oracle.padded_n(g.nu)
+    w = oracle.filter_weights("hann", n, float(np.float32(d[0, oracle.D_DU])), cutoff=0.5)
+    su = [oracle.start_u(g.nu, n, d[i, oracle.D_OFFU]) for i in range(first, first + count)]
+    f64 = oracle.filter_rows(q, w, n, su, method=1)
+    rng = np.random.default_rng(5)
+    idx = rng.choice(1024 ** 3, 4096, replace=False)
+    vol = oracle.volume(1024, 1024, 1024, 0.15, 0.15, 0.15)
+    ref = oracle.bp_fdk(c[first:first + count], f64, vol, idx)
+
+    full = {}
+    for storage, kahan in (("f32", False), ("f16", True)):
+        plan = yk.FdkPlan.from_geometry(ctx, g, filter="hann", storage=storage, kahan=kahan)
+        fs, ss = plan.filter(proj[first:first + count], first=first)
+        part = plan.backproject(fs, ss, first=first)
+        got = part.reshape(-1)[torch.from_numpy(idx).cuda()].cpu().numpy()
+        err = rel_rmse(got, ref)
+        del fs, ss, part
+        f, s = plan.filter(proj)
+        out = torch.empty(plan.volume_shape, device="cuda")
+        ms = timed_backprojection(plan, f, s, out)
+        del f, s
+        plan.close()
+        full[storage] = out
+        print(f"1024^3 x 1440 {storage}: 16-view slice vs FP64 rel-RMSE {err:.3e}; "
+              f"backprojection {ms:.1f} ms = {1024 ** 3 * 1440 / (ms * 1e-3) / 1e9:.0f} GUPS")
+        assert err < (1e-5 if storage == "f32" else 1e-3)
+    diff = float(torch.linalg.vector_norm(full["f16"] - full["f32"]) / torch.linalg.vector_norm(full["f32"]))
+    print(f"whole scan FP16 + Kahan vs FP32: rel {diff:.3e}")
+    assert diff < 1e-3
+
+
+def test_c5_cgls_10_iterations_through_autograd(c5):
+    g, P, truth, proj = c5
+    x, res = ag.cgls(P, proj, iterations=10)
+    err = float(torch.linalg.vector_norm(x - truth) / torch.linalg.vector_norm(truth))
+    print(f"CGLS 10 at 1024^3 x 1440: ||r|| {res[0]:.4e} -> {res[-1]:.4e}, rel error vs phantom {err:.3f}")
+    assert len(res) == 11
+    assert np.all(np.diff(res) <= 1e-6 * res[0])
+    assert res[-1] < 0.2 * res[0]
