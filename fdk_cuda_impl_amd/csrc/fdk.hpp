@@ -78,6 +78,9 @@ struct yk_projector {
     yk::DeviceArray<float2> d_sep_ab1; // separable Joseph transpose ray table, one view chunk
     yk::DeviceArray<float2> d_sep_ab2;
     yk::DeviceArray<float> d_sep_pl;
+    std::vector<char> siddon_separable;  // per view: detV along Z only
+    yk::DeviceArray<float2> d_sid_col;   // separable Siddon transpose ray table, one view chunk
+    yk::DeviceArray<float2> d_sid_ray;
     yk::DeviceArray<float> d_vol_t;
     yk::DeviceArray<double> d_partials;
     yk::DeviceArray<yk::OpBpViewDev> d_opviews;  // operator-form FDK backprojection records

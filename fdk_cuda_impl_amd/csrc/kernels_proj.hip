@@ -519,27 +519,34 @@ __device__ __forceinline__ Box volume_box(const VolumeGeom& g)
     return b;
 }
 
+// one slab of the box clip: narrows [t0, t1] to lo <= s + t r <= hi; a ray
+// parallel to the slab and outside it misses
+__device__ __forceinline__ bool slab_clip(float s, float r, float lo, float hi, float& t0, float& t1)
+{
+    if (fabsf(r) > 1e-8f) {
+        float ta = (lo - s) / r, tb = (hi - s) / r;
+        if (ta > tb) { const float t = ta; ta = tb; tb = t; }
+        t0 = fmaxf(t0, ta);
+        t1 = fminf(t1, tb);
+        return true;
+    }
+    return !(s < lo || s > hi);
+}
+
 // parametric segment [t0, t1] of src + t ray inside the box, clipped to [0, 1]
 __device__ __forceinline__ bool clip_ray(const float* src, const float* ray, const float* lo,
     const float* hi, float& t0, float& t1)
 {
     t0 = 0.f;
     t1 = 1.f;
-    for (int k = 0; k < 3; ++k) {
-        if (fabsf(ray[k]) > 1e-8f) {
-            float ta = (lo[k] - src[k]) / ray[k], tb = (hi[k] - src[k]) / ray[k];
-            if (ta > tb) { const float t = ta; ta = tb; tb = t; }
-            t0 = fmaxf(t0, ta);
-            t1 = fminf(t1, tb);
-        } else if (src[k] < lo[k] || src[k] > hi[k]) {
-            return false;
-        }
-    }
+    for (int k = 0; k < 3; ++k)
+        if (!slab_clip(src[k], ray[k], lo[k], hi[k], t0, t1)) return false;
     return t0 < t1;
 }
 
 __device__ __forceinline__ void siddon_ray_geom(const SiddonViewDev& sv, int u, int v, float* src, float* ray)
 {
+#pragma clang fp contract(off)
     for (int k = 0; k < 3; ++k) {
         src[k] = sv.src[k];
         ray[k] = (sv.detS[k] + float(u) * sv.detU[k] + float(v) * sv.detV[k]) - sv.src[k];
@@ -653,6 +660,168 @@ siddon_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
     }
     float* dst = volume + ((size_t)z * g.ny + y) * g.nx + x;
     *dst = accumulate ? *dst + acc : acc;
+}
+
+// ---------------------------------------------------------------------------
+// Siddon transpose for separable views (detV along Z only).  The rays of a
+// detector column u share ray_x and ray_y (independent of v), hence their XY
+// path: the t-interval of an (x, y) cell and the XY part of the volume clip
+// are column constants, and each ray only resolves the Z voxels it crosses
+// inside that interval.  Lane / slot layout of the separable Joseph transpose.
+// ---------------------------------------------------------------------------
+
+// col_xy[view][u] = (ray_x, ray_y) of column u; ray_wz[view][u][v] =
+// (p L, ray_z) of ray (u, v), L = |ray|, each column padded to nvp entries of
+// zeros (p L = 0 marks a ray that adds nothing)
+__global__ void __launch_bounds__(256)
+siddon_sep_table_kernel(const float* __restrict__ proj, const SiddonViewDev* __restrict__ views,
+    int nu, int nv, int nvp, float2* __restrict__ col_xy, float2* __restrict__ ray_wz)
+{
+    __shared__ float2 s_t[32][33];
+    const int i = blockIdx.z;
+    const int u0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const SiddonViewDev sv = views[i];
+    const float* pi = proj + (size_t)i * nu * nv;
+    for (int k = ty; k < 32; k += 8) {
+        const int u = u0 + tx, v = v0 + k;
+        float2 e = make_float2(0.f, 0.f);
+        if (u < nu && v < nv) {
+            float src[3], ray[3];
+            siddon_ray_geom(sv, u, v, src, ray);
+            const float len = sqrtf(ray[0] * ray[0] + ray[1] * ray[1] + ray[2] * ray[2]);
+            e = make_float2(pi[(size_t)v * nu + u] * len, ray[2]);
+            if (v == 0) col_xy[(size_t)i * nu + u] = make_float2(ray[0], ray[1]);
+        }
+        s_t[k][tx] = e;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int u = u0 + k, v = v0 + tx;
+        if (u < nu && v < nvp) ray_wz[((size_t)i * nu + u) * nvp + v] = s_t[tx][k];
+    }
+}
+
+// t-interval of the walk inside cell c of one axis: the forward walk's own
+// plane crossings (plane - src) * (1 / ray); a ray parallel to the axis stays
+// in the cell its source coordinate lies in
+__device__ __forceinline__ void cell_interval(float s, float r, float lo, float vox, int c, int n,
+    float& ta, float& tb)
+{
+    if (fabsf(r) > 1e-8f) {
+        const float rc = 1.f / r;
+        const float t0 = (lo + float(c) * vox - s) * rc, t1 = (lo + float(c + 1) * vox - s) * rc;
+        ta = fmaxf(ta, fminf(t0, t1));
+        tb = fminf(tb, fmaxf(t0, t1));
+    } else if (min(n - 1, max(0, int(floorf((s - lo) / vox)))) != c) {
+        tb = -3.0e38f;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+siddon_adjoint_sep_kernel(const float2* __restrict__ col_xy, const float2* __restrict__ ray_wz,
+    const SiddonViewDev* __restrict__ views, const float* __restrict__ c32, VolumeGeom g, int nu, int nv,
+    int nvp, int count, int accumulate, float* __restrict__ volume)
+{
+    __shared__ float slot[(kSepZ + 2) * 256];
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x * 16 + (lane & 15);
+    const int y = blockIdx.y * 16 + (lane >> 4);
+    const int z0 = blockIdx.z * kSepZ;
+#pragma unroll
+    for (int k = 0; k < kSepZ + 2; ++k) slot[k * 256 + lane] = 0.f;
+    if (x >= g.nx || y >= g.ny) return;
+    const Box b = volume_box(g);
+    const float margin = 0.05f;
+    const int zend = min(z0 + kSepZ, g.nz);  // one past the block's last slice
+    const float xl = b.lo[0] + float(x) * g.vx, xh = b.lo[0] + float(x + 1) * g.vx;
+    const float yl = b.lo[1] + float(y) * g.vy, yh = b.lo[1] + float(y + 1) * g.vy;
+    const float zl = b.lo[2] + float(z0) * g.vz, zh = b.lo[2] + float(zend) * g.vz;
+    const float inv_vz = 1.f / g.vz;
+    // only a block holding the volume's first or last slice can see the Z
+    // faces of the volume clip bind inside its voxels
+    const bool zface = z0 == 0 || zend == g.nz;
+    for (int i = 0; i < count; ++i) {
+        const float* c = c32 + (size_t)i * 12;
+        // detector range of the cell x block (8 corners, FP32 map, margin)
+        float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float X = (k & 1) ? xh : xl, Y = (k & 2) ? yh : yl, Z = (k & 4) ? zh : zl;
+            const float den = c[8] * X + c[9] * Y + c[10] * Z + c[11];
+            ok = ok && den > 1e-6f;
+            const float rd = 1.f / den;
+            const float uu = proj_coord(c, 0, X, Y, Z, rd), vv = proj_coord(c, 4, X, Y, Z, rd);
+            umin = fminf(umin, uu); umax = fmaxf(umax, uu);
+            vmin = fminf(vmin, vv); vmax = fmaxf(vmax, vv);
+        }
+        if (!ok) continue;
+        const int ulo = max(0, int(ceilf(umin - margin)));
+        const int uhi = min(nu - 1, int(floorf(umax + margin)));
+        const int vlo = max(0, int(ceilf(vmin - margin)));
+        const int vhi = min(nv - 1, int(floorf(vmax + margin)));
+        const SiddonViewDev sv = views[i];
+        const float sx = sv.src[0], sy = sv.src[1], sz = sv.src[2];
+        for (int u = ulo; u <= uhi; ++u) {
+            const float2 r = col_xy[(size_t)i * nu + u];
+            // XY part of the volume clip and of the cell, shared by the column
+            float a0 = 0.f, e0 = 1.f;
+            if (!slab_clip(sx, r.x, b.lo[0], b.hi[0], a0, e0)) continue;
+            if (!slab_clip(sy, r.y, b.lo[1], b.hi[1], a0, e0)) continue;
+            float ca = -3.0e38f, cb = 3.0e38f;
+            cell_interval(sx, r.x, b.lo[0], g.vx, x, g.nx, ca, cb);
+            cell_interval(sy, r.y, b.lo[1], g.vy, y, g.ny, ca, cb);
+            a0 = fmaxf(a0, ca);
+            e0 = fminf(e0, cb);
+            if (!(e0 > a0)) continue;
+            const float2* q = ray_wz + ((size_t)i * nu + u) * nvp + vlo;
+            const int nr = vhi - vlo + 1;
+            for (int r0 = 0; r0 < nr; r0 += kSepU) {
+                float2 e[kSepU];
+#pragma unroll
+                for (int j = 0; j < kSepU; ++j) e[j] = q[r0 + j];
+#pragma unroll
+                for (int j = 0; j < kSepU; ++j) {
+                    const float pw = e[j].x, rz = e[j].y;
+                    float a = a0, t1 = e0;
+                    bool live = pw != 0.f;
+                    if (zface) live = live && slab_clip(sz, rz, b.lo[2], b.hi[2], a, t1);
+                    live = live && t1 > a;
+                    if (!live) continue;
+                    // Z voxels of the block the ray crosses inside [a, t1]
+                    const bool zvalid = fabsf(rz) > 1e-8f;
+                    const float rc = zvalid ? 1.f / rz : 0.f;
+                    const float za = sz + a * rz, zb = sz + t1 * rz;
+                    int ka = int(floorf((fminf(za, zb) - b.lo[2]) * inv_vz));
+                    int kb = int(floorf((fmaxf(za, zb) - b.lo[2]) * inv_vz));
+                    ka = max(ka, z0);
+                    kb = min(kb, zend - 1);
+                    for (int k = ka; k <= kb; ++k) {
+                        float lo = a, hi = t1;
+                        if (zvalid) {
+                            const float tl = (b.lo[2] + float(k) * g.vz - sz) * rc;
+                            const float th = (b.lo[2] + float(k + 1) * g.vz - sz) * rc;
+                            lo = fmaxf(lo, fminf(tl, th));
+                            hi = fminf(hi, fmaxf(tl, th));
+                        }
+                        const float seg = hi - lo;
+                        if (seg > 0.f) slot[(k - z0 + 1) * 256 + lane] += pw * seg;
+                    }
+                }
+            }
+        }
+    }
+    const size_t plane = (size_t)g.nx * g.ny;
+    const size_t colx = (size_t)y * g.nx + x;
+#pragma unroll
+    for (int k = 1; k <= kSepZ; ++k) {
+        const int z = z0 + k - 1;
+        if (z >= g.nz) break;
+        const float val = slot[k * 256 + lane];
+        float* dst = volume + z * plane + colx;
+        *dst = accumulate ? *dst + val : val;
+    }
 }
 
 // Operator-form FDK backprojections, the iterative B of the reference's
@@ -772,6 +941,18 @@ void launch_siddon_fp(const SiddonLaunch& p, hipStream_t s)
     dim3 block(256), grid((p.nu + 15) / 16, (p.nv + 15) / 16, p.count);
     hipLaunchKernelGGL(siddon_fp_kernel, grid, block, 0, s, p.volume, p.proj, p.views, p.vol, p.nu,
         p.nv, p.accumulate);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_siddon_adjoint_sep(const SiddonSepLaunch& p, hipStream_t s)
+{
+    const int nvp = p.nv + kSepPad;
+    dim3 tb(256), tg((p.nu + 31) / 32, (nvp + 31) / 32, p.count);
+    hipLaunchKernelGGL(siddon_sep_table_kernel, tg, tb, 0, s, p.proj, p.views, p.nu, p.nv, nvp, p.col_xy,
+        p.ray_wz);
+    dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, (p.vol.nz + kSepZ - 1) / kSepZ);
+    hipLaunchKernelGGL(siddon_adjoint_sep_kernel, grid, block, 0, s, p.col_xy, p.ray_wz, p.views, p.coeff32,
+        p.vol, p.nu, p.nv, nvp, p.count, p.accumulate, p.volume);
     YK_HIP(hipGetLastError());
 }
 

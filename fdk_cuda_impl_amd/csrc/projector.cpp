@@ -45,6 +45,7 @@ yk_projector::yk_projector(yk_ctx* c, const yk_view* v, int n, int nu_, int nv_,
         // d0 and d1 of every ray independent of v (exact zeros, so bit-exact)
         separable.push_back(p.axis != 2 && p.detV[p.axis == 0 ? 0 : 1] == 0.f &&
                             p.detV[p.axis == 0 ? 1 : 0] == 0.f);
+        siddon_separable.push_back(w.detV[0] == 0.f && w.detV[1] == 0.f);
         SiddonViewDev& s = sv[i];
         std::memset(&s, 0, sizeof(s));
         std::memcpy(s.src, w.src, sizeof(s.src));
@@ -188,6 +189,29 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
             p.nu = nu; p.nv = nv; p.count = std::min(chunk, count - c0);
             p.accumulate = c0 > 0 ? 1 : accumulate;
             launch_joseph_adjoint(p, s);
+        }
+    } else if (kind == YK_BP_SIDDON_ADJOINT && std::all_of(siddon_separable.begin() + first,
+                                                            siddon_separable.begin() + first + count,
+                                                            [](char c) { return c != 0; })) {
+        // column-major ray table of a chunk of views at a time (at most 1 GiB)
+        const size_t per_view = size_t(nu) * nv;
+        const size_t table = size_t(nu) * (nv + kSepPad);
+        const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(count),
+            (size_t(1) << 30) / ((table + nu) * sizeof(float2)))));
+        d_sid_col.resize(std::max(d_sid_col.size(), size_t(nu) * chunk));
+        d_sid_ray.resize(std::max(d_sid_ray.size(), table * chunk));
+        for (int c0 = 0; c0 < count; c0 += chunk) {
+            SiddonSepLaunch p{};
+            p.proj = d_proj + per_view * c0;
+            p.volume = d_volume;
+            p.views = d_sviews.get() + first + c0;
+            p.coeff32 = d_c32.get() + size_t(first + c0) * 12;
+            p.col_xy = d_sid_col.get();
+            p.ray_wz = d_sid_ray.get();
+            p.vol = vol;
+            p.nu = nu; p.nv = nv; p.count = std::min(chunk, count - c0);
+            p.accumulate = c0 > 0 ? 1 : accumulate;
+            launch_siddon_adjoint_sep(p, s);
         }
     } else if (kind == YK_BP_SIDDON_ADJOINT) {
         SiddonAdjointLaunch p{};

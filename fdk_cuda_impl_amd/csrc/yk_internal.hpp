@@ -314,6 +314,21 @@ struct SiddonAdjointLaunch {
 };
 void launch_siddon_adjoint(const SiddonAdjointLaunch& p, hipStream_t s);
 
+// Transpose of the Siddon projector for separable views (detV along Z only);
+// the ray table arrays are scratch of `count` views.
+struct SiddonSepLaunch {
+    const float* proj;          // [count][nv][nu]
+    float* volume;              // [z][y][x]
+    const SiddonViewDev* views; // device, `count`
+    const float* coeff32;       // device, per view cu[4] cv[4] cd[4] (world projective map)
+    float2* col_xy;             // scratch [count][nu]
+    float2* ray_wz;             // scratch [count][nu][nv + kSepPad]
+    VolumeGeom vol;
+    int nu, nv, count;
+    int accumulate;
+};
+void launch_siddon_adjoint_sep(const SiddonSepLaunch& p, hipStream_t s);
+
 // per-view record of the operator-form FDK backprojections (world units)
 struct OpBpViewDev {
     float cu[4], cv[4], cd[4];  // projective map u = Cu.[X,1] / Cd.[X,1], v likewise
