@@ -82,6 +82,8 @@ struct yk_projector {
     yk::DeviceArray<float2> d_sid_col;   // separable Siddon transpose ray table, one view chunk
     yk::DeviceArray<float2> d_sid_ray;
     yk::DeviceArray<float> d_vol_t;
+    yk::DeviceArray<float> d_vol_zx;   // Z-fastest volume copies of the separable Joseph projection
+    yk::DeviceArray<float> d_vol_zy;
     yk::DeviceArray<double> d_partials;
     yk::DeviceArray<yk::OpBpViewDev> d_opviews;  // operator-form FDK backprojection records
     std::vector<double> op_scale;      // per first view: vx vy vz / (du dv)

@@ -188,6 +188,91 @@ joseph_fp_kernel(const float* __restrict__ vol, const float* __restrict__ vol_t,
     *dst = accumulate ? *dst + out : out;
 }
 
+// Joseph projection for separable views (main axis X or Y, detV along Z
+// only).  One wave per detector column (64 consecutive rays in v): a1, b1 and
+// hence f1 and its two first-minor taps are the same for every lane, and the
+// lanes' Z taps are neighbouring entries of a Z-fastest volume copy
+// ([x][y][z] for X-main, [y][x][z] for Y-main views), so the four gathers of
+// a sample touch a few cache lines per wave.  Sampling rule, slice range,
+// weights and summation order are those of joseph_fp_kernel (same results).
+__global__ void __launch_bounds__(256)
+joseph_fp_sep_kernel(const float* __restrict__ vzx, const float* __restrict__ vzy,
+    float* __restrict__ proj, const ProjViewDev* __restrict__ views, VolumeGeom g,
+    int nu, int nv, int accumulate)
+{
+    const int u = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int v = blockIdx.y * 64 + int(threadIdx.x & 63);
+    const int i = blockIdx.z;
+    if (u >= nu) return;  // whole wave
+    const bool pix = v < nv;
+    const ProjViewDev pv = views[i];
+    const AxisInfo ax = axis_info(pv.axis, g);
+    const float* base = pv.axis == 0 ? vzx : vzy;
+    JRay r{};
+    int lo = 1, hi = 0;
+    if (pix && joseph_ray(pv, ax, u, v, r)) joseph_slice_range(r, ax, lo, hi);
+    int wlo = lo <= hi ? lo : 0x7fffffff, whi = lo <= hi ? hi : -1;
+    for (int off = 32; off > 0; off >>= 1) {
+        wlo = min(wlo, __shfl_xor(wlo, off, 64));
+        whi = max(whi, __shfl_xor(whi, off, 64));
+    }
+    const int n1 = ax.n1, n2 = ax.n2;
+    float acc = 0.f;
+    constexpr int kUnroll = 4;
+    for (int s0 = wlo; s0 <= whi; s0 += kUnroll) {
+        float t[kUnroll][4], wa[kUnroll], wb[kUnroll], wc[kUnroll], wd[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+            const int s = min(s0 + k, whi);
+            float f1, f2;
+            joseph_sample(r, ax, s, f1, f2);
+            const float g1 = floorf(f1), g2 = floorf(f2);
+            const float w1 = f1 - g1, w2 = f2 - g2;
+            const bool live = s0 + k <= whi && s >= lo && s <= hi && g1 >= -1.f && g2 >= -1.f &&
+                g1 <= float(n1 - 1) && g2 <= float(n2 - 1);
+            const int j1 = live ? int(g1) : 0, j2 = live ? int(g2) : 0;
+            const bool a0 = j1 >= 0, a1 = j1 + 1 < n1, b0 = j2 >= 0, b1 = j2 + 1 < n2;
+            const float x0 = live && a0 ? 1.f - w1 : 0.f, x1 = live && a1 ? w1 : 0.f;
+            const float y0 = b0 ? 1.f - w2 : 0.f, y1 = b1 ? w2 : 0.f;
+            wa[k] = x0 * y0; wb[k] = x1 * y0; wc[k] = x0 * y1; wd[k] = x1 * y1;
+            const int c1 = max(j1, 0), c1b = min(j1 + 1, n1 - 1);
+            const int c2 = max(j2, 0), c2b = min(j2 + 1, n2 - 1);
+            const float* r0 = base + ((size_t)s * n1 + c1) * n2;
+            const float* r1 = base + ((size_t)s * n1 + c1b) * n2;
+            t[k][0] = r0[c2];
+            t[k][1] = r1[c2];
+            t[k][2] = r0[c2b];
+            t[k][3] = r1[c2b];
+        }
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k)
+            acc += (wa[k] * t[k][0] + wb[k] * t[k][1]) + (wc[k] * t[k][2] + wd[k] * t[k][3]);
+    }
+    if (!pix) return;
+    const float out = lo <= hi ? acc * r.L : 0.f;
+    float* dst = proj + ((size_t)i * nv + v) * nu + u;
+    *dst = accumulate ? *dst + out : out;
+}
+
+// [z][y][x] -> Z-fastest copy: out[x * sx + y * sy + z], one y row per
+// blockIdx.z, 32 x 32 (x, z) tiles through LDS
+__global__ void transpose_zfast_kernel(const float* __restrict__ in, float* __restrict__ out,
+    int nx, int ny, int nz, size_t sx, size_t sy)
+{
+    __shared__ float tile[32][33];
+    const int y = blockIdx.z;
+    const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int z = z0 + k, x = x0 + threadIdx.x;
+        if (x < nx && z < nz) tile[k][threadIdx.x] = in[((size_t)z * ny + y) * nx + x];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int x = x0 + k, z = z0 + threadIdx.x;
+        if (x < nx && z < nz) out[(size_t)x * sx + (size_t)y * sy + z] = tile[threadIdx.x][k];
+    }
+}
+
 // [z][y][x] -> [z][x][y]
 __global__ void transpose_xy_kernel(const float* __restrict__ in, float* __restrict__ out,
     int nx, int ny)
@@ -904,6 +989,23 @@ void launch_joseph_fp(const JosephLaunch& p, hipStream_t s)
     dim3 block(64, 4), grid((p.nu + 63) / 64, (p.nv + 3) / 4, p.count);
     hipLaunchKernelGGL(joseph_fp_kernel, grid, block, 0, s, p.volume, p.volume_t, p.proj, p.views,
         p.vol, p.nu, p.nv, p.accumulate);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_joseph_fp_sep(const JosephSepFpLaunch& p, hipStream_t s)
+{
+    dim3 block(256), grid((p.nu + 3) / 4, (p.nv + 63) / 64, p.count);
+    hipLaunchKernelGGL(joseph_fp_sep_kernel, grid, block, 0, s, p.vol_zx, p.vol_zy, p.proj, p.views, p.vol,
+        p.nu, p.nv, p.accumulate);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz, bool x_major, hipStream_t s)
+{
+    dim3 block(32, 8), grid((nx + 31) / 32, (nz + 31) / 32, ny);
+    const size_t sx = x_major ? size_t(ny) * nz : size_t(nz);
+    const size_t sy = x_major ? size_t(nz) : size_t(nx) * nz;
+    hipLaunchKernelGGL(transpose_zfast_kernel, grid, block, 0, s, in, out, nx, ny, nz, sx, sy);
     YK_HIP(hipGetLastError());
 }
 

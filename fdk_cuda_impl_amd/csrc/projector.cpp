@@ -107,7 +107,33 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
     if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
     if (!d_volume || !d_proj) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
     hipStream_t s = ctx->stream;
-    if (kind == YK_FP_JOSEPH) {
+    bool all_sep = true, any_x = false, any_y = false;
+    for (int i = first; i < first + count; ++i) {
+        all_sep = all_sep && separable[i];
+        const int ax = main_axis(views[i], nu, nv);
+        any_x = any_x || ax == 0;
+        any_y = any_y || ax == 1;
+    }
+    if (kind == YK_FP_JOSEPH && all_sep) {
+        const size_t nvox = size_t(grid.nx) * grid.ny * grid.nz;
+        if (any_x) {
+            d_vol_zx.resize(nvox);
+            launch_transpose_zfast(d_volume, d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
+        }
+        if (any_y) {
+            d_vol_zy.resize(nvox);
+            launch_transpose_zfast(d_volume, d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, s);
+        }
+        JosephSepFpLaunch p{};
+        p.vol_zx = any_x ? d_vol_zx.get() : nullptr;
+        p.vol_zy = any_y ? d_vol_zy.get() : nullptr;
+        p.proj = d_proj;
+        p.views = d_jviews.get() + first;
+        p.vol = vol;
+        p.nu = nu; p.nv = nv; p.count = count;
+        p.accumulate = accumulate;
+        launch_joseph_fp_sep(p, s);
+    } else if (kind == YK_FP_JOSEPH) {
         bool need_t = false;
         std::vector<ProjViewDev> dummy;
         for (int i = first; i < first + count; ++i)

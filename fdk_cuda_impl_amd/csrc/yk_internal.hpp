@@ -261,6 +261,20 @@ struct JosephLaunch {
 void launch_joseph_fp(const JosephLaunch& p, hipStream_t s);
 void launch_transpose_xy(const float* in, float* out, int nx, int ny, int nz, hipStream_t s);
 
+// Joseph projection for separable views from Z-fastest volume copies
+struct JosephSepFpLaunch {
+    const float* vol_zx;        // [x][y][z] copy (X-main views; may be null if none)
+    const float* vol_zy;        // [y][x][z] copy (Y-main views; may be null if none)
+    float* proj;                // [count][nv][nu]
+    const ProjViewDev* views;   // device, `count`
+    VolumeGeom vol;
+    int nu, nv, count;
+    int accumulate;
+};
+void launch_joseph_fp_sep(const JosephSepFpLaunch& p, hipStream_t s);
+// [z][y][x] -> [x][y][z] (x_major) or [y][x][z]
+void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz, bool x_major, hipStream_t s);
+
 struct JosephAdjointLaunch {
     const float* proj;          // [count][nv][nu]
     float* volume;              // [z][y][x]
