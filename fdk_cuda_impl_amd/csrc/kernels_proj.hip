@@ -688,6 +688,91 @@ siddon_fp_kernel(const float* __restrict__ vol, float* __restrict__ proj,
     *dst = accumulate ? *dst + acc : acc;
 }
 
+// Siddon projection for separable views (detV along Z only).  One wave per
+// detector column (64 consecutive rays in v): the rays share ray_x and
+// ray_y, so the XY cells they cross and the XY plane crossings are the same
+// for every lane and are walked once per wave; each lane walks its own Z
+// crossings inside every cell.  The events (XY and Z plane crossings, clip
+// ends) are those of siddon_fp_kernel, computed by the same expressions, so
+// the segments and their order match that walk.  Reads a Z-fastest
+// [x][y][z] volume copy, where the lanes of a wave touch neighbouring z.
+__global__ void __launch_bounds__(256)
+siddon_fp_sep_kernel(const float* __restrict__ vzx, float* __restrict__ proj,
+    const SiddonViewDev* __restrict__ views, VolumeGeom g, int nu, int nv, int accumulate)
+{
+    const int u = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int v = blockIdx.y * 64 + int(threadIdx.x & 63);
+    const int i = blockIdx.z;
+    if (u >= nu) return;  // whole wave
+    const bool pix = v < nv;
+    const SiddonViewDev sv = views[i];
+    float src[3], ray[3];
+    siddon_ray_geom(sv, u, pix ? v : 0, src, ray);
+    const Box b = volume_box(g);
+    // column part of the box clip (ray_x, ray_y are the same in every lane)
+    float a0 = 0.f, e0 = 1.f;
+    const bool colhit = slab_clip(src[0], ray[0], b.lo[0], b.hi[0], a0, e0) &&
+        slab_clip(src[1], ray[1], b.lo[1], b.hi[1], a0, e0);
+    float acc = 0.f;
+    float t0 = a0, t1 = e0;
+    const bool live = pix && colhit && slab_clip(src[2], ray[2], b.lo[2], b.hi[2], t0, t1) && t0 < t1;
+    if (colhit && a0 < e0) {
+        // XY walk state from the column's entry a0
+        int ix = min(g.nx - 1, max(0, int(floorf((src[0] + a0 * ray[0] - b.lo[0]) / g.vx))));
+        int iy = min(g.ny - 1, max(0, int(floorf((src[1] + a0 * ray[1] - b.lo[1]) / g.vy))));
+        const int sx = ray[0] >= 0.f ? 1 : -1, sy = ray[1] >= 0.f ? 1 : -1;
+        const bool vx_ok = fabsf(ray[0]) > 1e-8f, vy_ok = fabsf(ray[1]) > 1e-8f;
+        const float rcx = vx_ok ? 1.f / ray[0] : 0.f, rcy = vy_ok ? 1.f / ray[1] : 0.f;
+        float tnx = vx_ok ? (b.lo[0] + float(ix + (sx > 0 ? 1 : 0)) * g.vx - src[0]) * rcx : 3.0e38f;
+        float tny = vy_ok ? (b.lo[1] + float(iy + (sy > 0 ? 1 : 0)) * g.vy - src[1]) * rcy : 3.0e38f;
+        // Z walk state of this lane from its own entry t0
+        const int sz = ray[2] >= 0.f ? 1 : -1;
+        const bool vz_ok = fabsf(ray[2]) > 1e-8f;
+        const float rcz = vz_ok ? 1.f / ray[2] : 0.f;
+        int iz = 0;
+        float tnz = 3.0e38f;
+        if (live) {
+            iz = min(g.nz - 1, max(0, int(floorf((src[2] + t0 * ray[2] - b.lo[2]) / g.vz))));
+            tnz = vz_ok ? (b.lo[2] + float(iz + (sz > 0 ? 1 : 0)) * g.vz - src[2]) * rcz : 3.0e38f;
+        }
+        bool zin = live;  // lane still inside the volume in z
+        float t = a0;
+        const int max_steps = g.nx + g.ny + 2;
+        for (int n = 0; n < max_steps && t < e0; ++n) {
+            const float tc = fminf(fminf(tnx, tny), e0);  // end of the current XY cell
+            // this lane's part of the cell, split at its Z crossings
+            float la = fmaxf(t, t0);
+            const float lb = fminf(tc, t1);
+            const float* col = vzx + ((size_t)ix * g.ny + iy) * g.nz;
+            while (zin && la < lb) {
+                const float te = fminf(tnz, lb);
+                acc += col[iz] * (te - la);
+                la = te;
+                if (tnz <= lb) {
+                    iz += sz;
+                    if (iz < 0 || iz >= g.nz) { zin = false; break; }
+                    tnz = (b.lo[2] + float(iz + (sz > 0 ? 1 : 0)) * g.vz - src[2]) * rcz;
+                }
+            }
+            t = tc;
+            if (tnx <= tny) {
+                ix += sx;
+                if (ix < 0 || ix >= g.nx) break;
+                tnx = (b.lo[0] + float(ix + (sx > 0 ? 1 : 0)) * g.vx - src[0]) * rcx;
+            } else {
+                iy += sy;
+                if (iy < 0 || iy >= g.ny) break;
+                tny = (b.lo[1] + float(iy + (sy > 0 ? 1 : 0)) * g.vy - src[1]) * rcy;
+            }
+        }
+    }
+    if (!pix) return;
+    const float len = sqrtf(ray[0] * ray[0] + ray[1] * ray[1] + ray[2] * ray[2]);
+    const float out = live ? acc * len : 0.f;
+    float* dst = proj + ((size_t)i * nv + v) * nu + u;
+    *dst = accumulate ? *dst + out : out;
+}
+
 // Siddon transpose: one thread per voxel; for each view the candidate rays
 // are the pixels inside the projection of the voxel cube, and each candidate
 // contributes y * |segment inside the cube, clipped like the forward walk|.
@@ -1006,6 +1091,14 @@ void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz,
     const size_t sx = x_major ? size_t(ny) * nz : size_t(nz);
     const size_t sy = x_major ? size_t(nz) : size_t(nx) * nz;
     hipLaunchKernelGGL(transpose_zfast_kernel, grid, block, 0, s, in, out, nx, ny, nz, sx, sy);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_siddon_fp_sep(const SiddonSepFpLaunch& p, hipStream_t s)
+{
+    dim3 block(256), grid((p.nu + 3) / 4, (p.nv + 63) / 64, p.count);
+    hipLaunchKernelGGL(siddon_fp_sep_kernel, grid, block, 0, s, p.vol_zx, p.proj, p.views, p.vol, p.nu, p.nv,
+        p.accumulate);
     YK_HIP(hipGetLastError());
 }
 

@@ -151,6 +151,19 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
         p.nu = nu; p.nv = nv; p.count = count;
         p.accumulate = accumulate;
         launch_joseph_fp(p, s);
+    } else if (kind == YK_FP_SIDDON && std::all_of(siddon_separable.begin() + first,
+                                                   siddon_separable.begin() + first + count,
+                                                   [](char c) { return c != 0; })) {
+        d_vol_zx.resize(size_t(grid.nx) * grid.ny * grid.nz);
+        launch_transpose_zfast(d_volume, d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
+        SiddonSepFpLaunch p{};
+        p.vol_zx = d_vol_zx.get();
+        p.proj = d_proj;
+        p.views = d_sviews.get() + first;
+        p.vol = vol;
+        p.nu = nu; p.nv = nv; p.count = count;
+        p.accumulate = accumulate;
+        launch_siddon_fp_sep(p, s);
     } else if (kind == YK_FP_SIDDON) {
         SiddonLaunch p{};
         p.volume = d_volume;

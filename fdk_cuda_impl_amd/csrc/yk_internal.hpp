@@ -316,6 +316,16 @@ struct SiddonLaunch {
     int accumulate;
 };
 void launch_siddon_fp(const SiddonLaunch& p, hipStream_t s);
+// Siddon projection for separable views from the [x][y][z] volume copy
+struct SiddonSepFpLaunch {
+    const float* vol_zx;        // [x][y][z]
+    float* proj;                // [count][nv][nu]
+    const SiddonViewDev* views; // device, `count`
+    VolumeGeom vol;
+    int nu, nv, count;
+    int accumulate;
+};
+void launch_siddon_fp_sep(const SiddonSepFpLaunch& p, hipStream_t s);
 
 struct SiddonAdjointLaunch {
     const float* proj;
