@@ -18,9 +18,10 @@ CSRC = os.path.join(HERE, "csrc")
 REPO = os.path.dirname(HERE)
 OBJ_DIR = os.path.join(REPO, "_build", "obj")
 LIB_PATH = os.path.join(HERE, "libyk_hip.so")
+SESSION_CHECK = os.path.join(HERE, "yk_session_check")
 ARCH = os.environ.get("YK_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["capi.cpp", "geometry.cpp", "fdk.cpp", "projector.cpp",
+SOURCES = ["capi.cpp", "geometry.cpp", "fdk.cpp", "projector.cpp", "session.cpp",
            "kernels_filter.hip", "kernels_bp.hip", "kernels_proj.hip", "kernels_misc.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -61,7 +62,23 @@ def build(jobs: int | None = None) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB_PATH)
+    build_session_check()
     return LIB_PATH
+
+
+def build_session_check() -> str:
+    """Compile tests/session_check.cpp (the Session-API scenarios) against
+    include/yk_session.hpp and libyk_hip.so, next to the library."""
+    src = os.path.join(REPO, "tests", "session_check.cpp")
+    out = SESSION_CHECK
+    cmd = [hipcc(), "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-I", os.path.join(REPO, "include"),
+           "-x", "hip", src, "-o", out + ".tmp", "-L", HERE, "-lyk_hip", f"-Wl,-rpath,{HERE}",
+           "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"session check build failed\n{res.stdout}\n{res.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -1,0 +1,381 @@
+// C++ Session facade (include/yk_session.hpp) over this library's C ABI: the
+// reference's ReconstructionSession (src/interface/YkReconstructionApi.cpp:
+// 133-418) with its FDK and ForwardProjection backends
+// (src/interface/YkSession.cpp:693-728, 881-935,
+// src/common/YkProjectionOperators.hpp:138-180) for flat detectors on
+// circular scans.  Validation order and error codes follow the reference:
+// struct_size / api_version, enum ranges, projector selection, pose, then
+// the backend; execute checks the streaming contract and the buffer
+// capacities before any GPU work.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/yk_hip.h"
+#include "../../include/yk_session.hpp"
+
+namespace YK {
+namespace {
+
+bool has_bytes(uint32_t actual, size_t required) { return actual >= required; }
+
+bool valid_location(EMemoryLocation l) { return l == EMemoryLocation::Host || l == EMemoryLocation::Device; }
+
+// 3 x 3 rotation R = Rz Ry Rx (fixed world axes X, then Y, then Z), FP64
+void euler_rotation(double rx, double ry, double rz, double R[3][3])
+{
+    const double cx = std::cos(rx), sx = std::sin(rx), cy = std::cos(ry), sy = std::sin(ry);
+    const double cz = std::cos(rz), sz = std::sin(rz);
+    const double X[3][3] = { { 1, 0, 0 }, { 0, cx, -sx }, { 0, sx, cx } };
+    const double Y[3][3] = { { cy, 0, sy }, { 0, 1, 0 }, { -sy, 0, cy } };
+    const double Z[3][3] = { { cz, -sz, 0 }, { sz, cz, 0 }, { 0, 0, 1 } };
+    double ZY[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            ZY[i][j] = 0.0;
+            for (int k = 0; k < 3; ++k) ZY[i][j] += Z[i][k] * Y[k][j];
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            R[i][j] = 0.0;
+            for (int k = 0; k < 3; ++k) R[i][j] += ZY[i][k] * X[k][j];
+        }
+}
+
+// Views in the frame of a volume rotated by R about c: points map by
+// R^T (p - c) + c, directions by R^T (the inverse rigid transform applied to
+// the cone vectors, as transformProjectionGeometry does with local_from_world).
+void rotate_views(std::vector<yk_view>& views, const SForwardProjectionPose& pose, const float c[3])
+{
+    double R[3][3];
+    euler_rotation(pose.rotation_x_rad, pose.rotation_y_rad, pose.rotation_z_rad, R);
+    auto vec = [&](float* v) {
+        double o[3];
+        for (int i = 0; i < 3; ++i) o[i] = R[0][i] * v[0] + R[1][i] * v[1] + R[2][i] * v[2];
+        for (int i = 0; i < 3; ++i) v[i] = float(o[i]);
+    };
+    auto point = [&](float* p) {
+        double d[3], o[3];
+        for (int i = 0; i < 3; ++i) d[i] = double(p[i]) - c[i];
+        for (int i = 0; i < 3; ++i) o[i] = R[0][i] * d[0] + R[1][i] * d[1] + R[2][i] * d[2] + c[i];
+        for (int i = 0; i < 3; ++i) p[i] = float(o[i]);
+    };
+    for (auto& v : views) {
+        point(v.src);
+        point(v.detS);
+        vec(v.detU);
+        vec(v.detV);
+    }
+}
+
+// Scoped switch to the session's device, restoring the caller's.
+class DeviceScope {
+public:
+    explicit DeviceScope(int device)
+    {
+        ok_ = hipGetDevice(&previous_) == hipSuccess && hipSetDevice(device) == hipSuccess;
+    }
+    ~DeviceScope()
+    {
+        if (ok_) (void)hipSetDevice(previous_);
+    }
+    bool ok() const { return ok_; }
+
+private:
+    int previous_ = 0;
+    bool ok_ = false;
+};
+
+class Session final : public IReconstructionSession {
+public:
+    ~Session() override { release(); }
+
+    bool initialize(const SSystemSpec& spec) override
+    {
+        release();
+        clear();
+        if (!has_bytes(spec.struct_size, offsetof(SSystemSpec, device) + sizeof(spec.device)))
+            return fail(EApiErrorCode::InvalidConfig, "SSystemSpec is truncated or from an older layout");
+        if (spec.api_version != 2) return fail(EApiErrorCode::InvalidConfig, "unsupported SSystemSpec api_version");
+        const SSystemConfig& g = spec.geometry;
+        const SReconstructionSpec& rec = spec.reconstruction;
+        const int det = int(g.detector), traj = int(g.trajectory), pipe = int(rec.pipeline);
+        if (det < 0 || det > 1 || traj < 0 || traj > 3 || pipe < 0 || pipe > int(EPipeline::Statistical))
+            return fail(EApiErrorCode::InvalidConfig, "pipeline, detector or trajectory enum out of range");
+        const int fwd = int(rec.forward_projector), back = int(rec.back_projector);
+        if (fwd < 0 || fwd > int(EForwardProjector::SeparableFootprint) || back < 0 ||
+            back > int(EBackProjector::SeparableFootprint))
+            return fail(EApiErrorCode::InvalidConfig, "forward or back projector enum out of range");
+        if (g.detector != EDetectorKind::Flat || g.trajectory != ETrajectoryKind::Circular)
+            return fail(EApiErrorCode::UnsupportedCombination,
+                        "this library provides the flat-detector circular-scan path only");
+        if (rec.pipeline != EPipeline::FDK && rec.pipeline != EPipeline::ForwardProjection)
+            return fail(EApiErrorCode::UnsupportedCombination,
+                        "the Session facade runs the FDK and ForwardProjection pipelines");
+        const SForwardProjectionPose& pose = g.forward_projection_pose;
+        if (!std::isfinite(pose.rotation_x_rad) || !std::isfinite(pose.rotation_y_rad) ||
+            !std::isfinite(pose.rotation_z_rad))
+            return fail(EApiErrorCode::InvalidGeometry, "forward-projection volume rotation must be finite radians");
+        if (rec.pipeline == EPipeline::ForwardProjection) {
+            // Automatic is Siddon for flat detectors (resolveForwardProjector,
+            // src/interface/YkProjectorSelection.hpp:11-23)
+            if (rec.forward_projector == EForwardProjector::SeparableFootprint)
+                return fail(EApiErrorCode::UnsupportedCombination, "the separable-footprint projector is not provided");
+            forward_kind_ = rec.forward_projector == EForwardProjector::Joseph ? YK_FP_JOSEPH : YK_FP_SIDDON;
+        } else {
+            const int p = int(rec.fdk.parker.mode), f = int(rec.fdk.filter.type);
+            if (p < 0 || p > 2 || f < 0 || f > 9)
+                return fail(EApiErrorCode::InvalidConfig, "FDK Parker mode or filter enum out of range");
+        }
+
+        yk_system sys{};
+        const SRegularCircularScanSpec& c = g.circular;
+        sys.scan.total_views = c.total_views;
+        sys.scan.views_per_turn = c.views_per_turn;
+        sys.scan.start_angle_rad = c.start_angle_rad;
+        sys.scan.rotation_direction = c.rotation_direction;
+        sys.scan.sid_mm = c.sid_mm;
+        sys.scan.sdd_mm = c.sdd_mm;
+        sys.scan.z_mm = c.z_mm;
+        sys.scan.source_offset_mm[0] = c.source_offset_mm.x;
+        sys.scan.source_offset_mm[1] = c.source_offset_mm.y;
+        sys.scan.source_offset_mm[2] = c.source_offset_mm.z;
+        const SFlatDetectorSpec& d = g.flat_detector;
+        sys.detector.channels = d.channels;
+        sys.detector.rows = d.rows;
+        sys.detector.channel_size_mm = d.channel_size_mm;
+        sys.detector.row_size_mm = d.row_size_mm;
+        sys.detector.pose.offset_unv_mm[0] = d.pose.offset_unv_mm.x;
+        sys.detector.pose.offset_unv_mm[1] = d.pose.offset_unv_mm.y;
+        sys.detector.pose.offset_unv_mm[2] = d.pose.offset_unv_mm.z;
+        sys.detector.pose.tilt_u_rad = d.pose.tilt_u_rad;
+        sys.detector.pose.tilt_v_rad = d.pose.tilt_v_rad;
+        sys.detector.pose.tilt_n_rad = d.pose.tilt_n_rad;
+        const SVolumeGridSpec& v = g.volume;
+        sys.volume.nx = v.nx;
+        sys.volume.ny = v.ny;
+        sys.volume.nz = v.nz;
+        sys.volume.voxel_x_mm = v.voxel_x_mm;
+        sys.volume.voxel_y_mm = v.voxel_y_mm;
+        sys.volume.voxel_z_mm = v.voxel_z_mm;
+        sys.volume.center_mm[0] = v.center_mm.x;
+        sys.volume.center_mm[1] = v.center_mm.y;
+        sys.volume.center_mm[2] = v.center_mm.z;
+        if (v.nx <= 0 || v.ny <= 0 || v.nz <= 0 || d.channels <= 0 || d.rows <= 0)
+            return fail(EApiErrorCode::InvalidGeometry, "volume and detector sizes must be positive");
+
+        // first call: validate the scan and count its views
+        int n = 0;
+        if (yk_status st = yk_geometry_build(&sys, nullptr, 0, &n); st != YK_OK && st != YK_ERR_INVALID_BUFFER)
+            return fail_status(st, "geometry");
+        if (n <= 0) return fail(EApiErrorCode::InvalidGeometry, "the scan has no views");
+        std::vector<yk_view> views(static_cast<size_t>(n));
+        if (yk_status st = yk_geometry_build(&sys, views.data(), n, &n); st != YK_OK)
+            return fail_status(st, "geometry");
+
+        device_ = spec.device;
+        if (yk_status st = yk_ctx_create(device_, &ctx_); st != YK_OK) return fail_status(st, "device context");
+        if (rec.pipeline == EPipeline::FDK) {
+            yk_fdk_desc desc{};
+            desc.struct_size = sizeof(desc);
+            const SAnalyticFilterParams& f = rec.fdk.filter;
+            desc.filter.type = int(f.type);
+            desc.filter.cutoff = f.cutoff;
+            desc.filter.gain = f.gain;
+            desc.filter.butterworth_order = f.butterworth_order;
+            desc.filter.kaiser_beta = f.kaiser_beta;
+            desc.filter.tukey_alpha = f.tukey_alpha;
+            desc.parker_mode = int(rec.fdk.parker.mode);
+            desc.rotation_direction = c.rotation_direction;
+            desc.storage = YK_F32;
+            if (yk_status st = yk_fdk_create_system(ctx_, &desc, &sys, &fdk_); st != YK_OK) {
+                const bool r = fail_status(st, "FDK plan");
+                release_handles();
+                return r;
+            }
+        } else {
+            if (pose.enabled) rotate_views(views, pose, sys.volume.center_mm);
+            const float range = float(2.0 * 3.14159265358979323846 * c.total_views / c.views_per_turn);
+            if (yk_status st = yk_projector_create(ctx_, views.data(), n, d.channels, d.rows, &sys.volume, range,
+                                                   &proj_);
+                st != YK_OK) {
+                const bool r = fail_status(st, "projector");
+                release_handles();
+                return r;
+            }
+        }
+        pipeline_ = rec.pipeline;
+        views_ = n;
+        view_elems_ = size_t(d.channels) * d.rows;
+        vol_elems_ = size_t(v.nx) * v.ny * v.nz;
+        received_ = 0;
+        initialized_ = true;
+        return true;
+    }
+
+    bool execute(const SExecutionRequest& r) override
+    {
+        clear();
+        if (!initialized_) return fail(EApiErrorCode::StreamingStateError, "the session is not initialized");
+        if (!has_bytes(r.struct_size, offsetof(SExecutionRequest, iteration_count) + sizeof(r.iteration_count)))
+            return fail(EApiErrorCode::InvalidConfig, "SExecutionRequest is truncated or from an older layout");
+        if (r.struct_size > offsetof(SExecutionRequest, progress_callback) &&
+            !has_bytes(r.struct_size, offsetof(SExecutionRequest, progress_user_data) + sizeof(r.progress_user_data)))
+            return fail(EApiErrorCode::InvalidConfig, "SExecutionRequest progress tail is truncated");
+        if (r.api_version != 2) return fail(EApiErrorCode::InvalidConfig, "unsupported SExecutionRequest api_version");
+        if (!r.projection.data || !r.volume.data || !valid_location(r.projection.location) ||
+            !valid_location(r.volume.location))
+            return fail(EApiErrorCode::InvalidBuffer, "projection / volume buffer is null or its location is invalid");
+        const int offset = r.view_offset;
+        const int count = r.view_count > 0 ? r.view_count : views_;
+        if (offset < 0 || count <= 0 || offset > views_ - count)
+            return fail(EApiErrorCode::InvalidConfig, "view_offset / view_count outside the prepared views");
+        if (pipeline_ == EPipeline::FDK) {
+            if (offset != received_)
+                return fail(EApiErrorCode::StreamingStateError,
+                            "FDK batches must arrive in order: view_offset must equal the views received");
+            if (received_ > 0 && r.clear_output)
+                return fail(EApiErrorCode::StreamingStateError, "only the first FDK batch may clear the output");
+        } else if (offset != 0 || count != views_) {
+            return fail(EApiErrorCode::StreamingStateError,
+                        "this pipeline takes the whole view sequence from view_offset 0");
+        }
+        if (r.projection.element_count < view_elems_ * size_t(count) || r.volume.element_count < vol_elems_)
+            return fail(EApiErrorCode::InvalidBuffer, "projection / volume element_count below what this call needs");
+
+        const bool ok = pipeline_ == EPipeline::FDK ? execute_fdk(r, offset, count) : execute_fp(r);
+        if (!ok) return false;
+        if (pipeline_ == EPipeline::FDK) received_ += count;
+        const bool tail = has_bytes(r.struct_size, offsetof(SExecutionRequest, progress_user_data) +
+                                                       sizeof(r.progress_user_data));
+        if (tail && r.progress_callback)
+            r.progress_callback(r.progress_user_data, pipeline_ == EPipeline::FDK ? received_ : views_, views_);
+        return true;
+    }
+
+    bool isInitialized() const override { return initialized_; }
+    EApiErrorCode lastError() const override { return code_; }
+    const char* lastErrorMessage() const override { return message_.c_str(); }
+
+    void reset() override
+    {
+        if (fdk_) (void)yk_fdk_reset(fdk_);
+        received_ = 0;
+    }
+
+    void release() override
+    {
+        release_handles();
+        views_ = received_ = 0;
+        view_elems_ = vol_elems_ = 0;
+        initialized_ = false;
+    }
+
+private:
+    bool execute_fdk(const SExecutionRequest& r, int offset, int count)
+    {
+        const yk_status st = yk_fdk_execute(fdk_, r.projection.data, int(r.projection.location),
+            r.projection.element_count, offset, count, r.volume.data, int(r.volume.location),
+            r.volume.element_count, r.clear_output ? 1 : 0);
+        return st == YK_OK ? true : fail_status(st, "FDK");
+    }
+
+    // staging buffers for host-side FP inputs and outputs
+    float* device_buffer(void*& p, size_t& have, size_t n)
+    {
+        if (have < n) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            have = 0;
+            if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) return nullptr;
+            have = n;
+        }
+        return static_cast<float*>(p);
+    }
+
+    bool execute_fp(const SExecutionRequest& r)
+    {
+        DeviceScope scope(device_);
+        if (!scope.ok()) return fail(EApiErrorCode::CudaError, "cannot select the session's device");
+        const size_t np = view_elems_ * size_t(views_);
+        const float* d_vol = r.volume.data;
+        if (r.volume.location == EMemoryLocation::Host) {
+            float* s = device_buffer(d_vol_, vol_have_, vol_elems_);
+            if (!s || hipMemcpy(s, r.volume.data, vol_elems_ * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+                return fail(EApiErrorCode::CudaError, "volume upload failed");
+            d_vol = s;
+        }
+        float* d_proj = r.projection.data;
+        if (r.projection.location == EMemoryLocation::Host) {
+            d_proj = device_buffer(d_proj_, proj_have_, np);
+            if (!d_proj) return fail(EApiErrorCode::CudaError, "projection staging allocation failed");
+        }
+        if (yk_status st = yk_project(proj_, forward_kind_, d_vol, d_proj, 0, views_, 0); st != YK_OK)
+            return fail_status(st, "forward projection");
+        if (r.projection.location == EMemoryLocation::Host &&
+            hipMemcpy(r.projection.data, d_proj, np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(EApiErrorCode::CudaError, "projection download failed");
+        return true;
+    }
+
+    void release_handles()
+    {
+        if (fdk_) (void)yk_fdk_destroy(fdk_);
+        if (proj_) (void)yk_projector_destroy(proj_);
+        if (d_vol_ || d_proj_) {
+            DeviceScope scope(device_);
+            if (d_vol_) (void)hipFree(d_vol_);
+            if (d_proj_) (void)hipFree(d_proj_);
+        }
+        if (ctx_) (void)yk_ctx_destroy(ctx_);
+        fdk_ = nullptr;
+        proj_ = nullptr;
+        ctx_ = nullptr;
+        d_vol_ = d_proj_ = nullptr;
+        vol_have_ = proj_have_ = 0;
+    }
+
+    bool fail(EApiErrorCode c, const std::string& m)
+    {
+        code_ = c;
+        message_ = m;
+        return false;
+    }
+    bool fail_status(yk_status st, const char* what)
+    {
+        return fail(EApiErrorCode(int(st)), std::string(what) + ": " + yk_last_error());
+    }
+    void clear()
+    {
+        code_ = EApiErrorCode::None;
+        message_.clear();
+    }
+
+    yk_ctx* ctx_ = nullptr;
+    yk_fdk* fdk_ = nullptr;
+    yk_projector* proj_ = nullptr;
+    EPipeline pipeline_ = EPipeline::FDK;
+    int forward_kind_ = YK_FP_SIDDON;
+    int device_ = 0;
+    int views_ = 0;
+    int received_ = 0;
+    size_t view_elems_ = 0, vol_elems_ = 0;
+    void* d_vol_ = nullptr;
+    void* d_proj_ = nullptr;
+    size_t vol_have_ = 0, proj_have_ = 0;
+    bool initialized_ = false;
+    EApiErrorCode code_ = EApiErrorCode::None;
+    std::string message_;
+};
+
+}  // namespace
+
+IReconstructionSession* ReconstructionSessionFactory::create() { return new (std::nothrow) Session(); }
+
+void ReconstructionSessionFactory::destroy(IReconstructionSession* session) { delete session; }
+
+}  // namespace YK

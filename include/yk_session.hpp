@@ -1,0 +1,192 @@
+// C++ Session API of the flat circular path, source-compatible with the
+// reference's public interface for the FDK and ForwardProjection pipelines:
+//   include/YKCBCT/interface/YkReconstructionApi.hpp:12-98 (SSystemSpec,
+//     SExecutionRequest, EApiErrorCode, IReconstructionSession,
+//     ReconstructionSessionFactory),
+//   include/YKCBCT/interface/YkReconstructionTypes.hpp:14-44 (EPipeline,
+//     EMemoryLocation, Buffer),
+//   include/YKCBCT/interface/YkSystemReconstruction.hpp:8-18 (SReconstructionSpec),
+//   include/YKCBCT/geometry/YkSystemGeometry.hpp:12-31 and
+//     YkGeometryBuilderTypes.hpp:11-117 (SSystemConfig and its scan, detector,
+//     volume and pose records),
+//   include/YKCBCT/algorithms/analytic/YkFdkParams.hpp, YkAnalyticFilterParams.hpp,
+//   include/YKCBCT/algorithms/projection/YkProjectionParams.hpp.
+// Field names, order, defaults and meanings follow those headers; records of
+// pipelines this library does not provide (helical, cylindrical, iterative
+// parameter blocks, statistical and plugin tails) are omitted, and such
+// requests are refused with UnsupportedCombination.  float3 is HIP's.
+//
+// The implementation (ReconstructionSessionFactory in libyk_hip.so) sits on
+// the C ABI of yk_hip.h.  Like the reference, a session is not thread-safe,
+// execute() is synchronous on success, buffers are borrowed for the call and
+// element_count is a float count.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include <hip/hip_vector_types.h>
+
+namespace YK {
+
+// --- geometry ---------------------------------------------------------------
+enum class EDetectorKind : int32_t { Flat, Cylindrical };
+enum class ETrajectoryKind : int32_t { Circular, Helical, PlanarSourceOrbit, NikonTilted };
+
+// Regular static circular scan: total_views < views_per_turn is a short scan.
+struct SRegularCircularScanSpec {
+    int total_views = 0;
+    int views_per_turn = 0;
+    float start_angle_rad = 0.f;
+    int rotation_direction = 1;  // +1 / -1
+    float sid_mm = 0.f;
+    float sdd_mm = 0.f;
+    float z_mm = 0.f;
+    float3 source_offset_mm = make_float3(0.f, 0.f, 0.f);
+};
+
+// Detector pose relative to the nominal principal point: offsets along
+// U, N, V; tilts (rad) about U, then V, then N.  U x V = N, N toward the source.
+struct SDetectorPoseSpec {
+    float3 offset_unv_mm = make_float3(0.f, 0.f, 0.f);
+    float tilt_u_rad = 0.f;
+    float tilt_v_rad = 0.f;
+    float tilt_n_rad = 0.f;
+};
+
+struct SFlatDetectorSpec {
+    int channels = 0;
+    int rows = 0;
+    float channel_size_mm = 0.f;
+    float row_size_mm = 0.f;
+    SDetectorPoseSpec pose{};
+};
+
+struct SVolumeGridSpec {
+    int nx = 0;
+    int ny = 0;
+    int nz = 0;
+    float voxel_x_mm = 0.f;
+    float voxel_y_mm = 0.f;
+    float voxel_z_mm = 0.f;
+    float3 center_mm = make_float3(0.f, 0.f, 0.f);
+};
+
+// Volume rotation of the standalone forward projection: fixed world-axis X,
+// then Y, then Z rotations about the volume centre.
+struct SForwardProjectionPose {
+    bool enabled = false;
+    float rotation_x_rad = 0.f;
+    float rotation_y_rad = 0.f;
+    float rotation_z_rad = 0.f;
+};
+
+struct SSystemConfig {
+    EDetectorKind detector = EDetectorKind::Flat;
+    ETrajectoryKind trajectory = ETrajectoryKind::Circular;
+    SRegularCircularScanSpec circular{};
+    SFlatDetectorSpec flat_detector{};
+    SVolumeGridSpec volume{};
+    SForwardProjectionPose forward_projection_pose{};
+};
+
+// --- algorithm --------------------------------------------------------------
+enum class EPipeline : int32_t {
+    FDK, ForwardProjection, SIRT, OSSART, CGLS, PWLS, XFDK, CFDK, WFBP, TigreGradient, SART, Statistical,
+};
+
+enum class EForwardProjector : int32_t { Automatic, Siddon, Joseph, SeparableFootprint };
+enum class EBackProjector : int32_t {
+    Automatic, FdkPseudoMatched, SiddonRayDriven, Joseph, JosephV3, SeparableFootprint,
+};
+
+enum class EParkerMode : int32_t { Auto, Disabled, Enabled };
+struct SParkerScanSpec {
+    EParkerMode mode = EParkerMode::Auto;
+};
+
+enum class EFdkFilter : int32_t {
+    None = 0, RamLak = 1, SheppLogan = 2, Cosine = 3, Hann = 4, Hamming = 5,
+    Blackman = 6, Butterworth = 7, Kaiser = 8, Tukey = 9
+};
+struct SAnalyticFilterParams {
+    EFdkFilter type = EFdkFilter::RamLak;
+    float cutoff = .5f, gain = 1.f;
+    float butterworth_order = 2.f, kaiser_beta = 8.6f, tukey_alpha = .5f;
+};
+
+struct SFdkAlgoParams {
+    SParkerScanSpec parker{};
+    SAnalyticFilterParams filter{};
+};
+
+struct SReconstructionSpec {
+    EPipeline pipeline = EPipeline::FDK;
+    EForwardProjector forward_projector = EForwardProjector::Automatic;
+    EBackProjector back_projector = EBackProjector::Automatic;
+    SFdkAlgoParams fdk{};
+};
+
+// --- session ----------------------------------------------------------------
+enum class EMemoryLocation : int32_t { Host, Device };
+
+// Contiguous float arrays: projection [view][v][u], volume [z][y][x].
+struct Buffer {
+    float* data = nullptr;
+    EMemoryLocation location = EMemoryLocation::Device;
+    size_t element_count = 0;  // capacity in floats
+};
+
+struct SSystemSpec {
+    uint32_t struct_size = sizeof(SSystemSpec);
+    uint32_t api_version = 2;
+    SSystemConfig geometry{};
+    SReconstructionSpec reconstruction{};
+    int device = 0;
+};
+
+struct SExecutionRequest {
+    uint32_t struct_size = sizeof(SExecutionRequest);
+    uint32_t api_version = 2;
+    Buffer projection{};
+    Buffer volume{};
+    int view_offset = 0;
+    int view_count = 0;  // 0: every prepared view
+    bool clear_output = true;
+    int iteration_count = 0;
+    // Optional tail, called on execute's thread after the GPU work of a
+    // successful call (completed, total = views received so far, views prepared).
+    void (*progress_callback)(void*, int completed, int total) = nullptr;
+    void* progress_user_data = nullptr;
+};
+
+enum class EApiErrorCode : int32_t {
+    None = 0,
+    InvalidConfig,
+    InvalidGeometry,
+    UnsupportedCombination,
+    AlgorithmUnderTest,
+    InvalidBuffer,
+    StreamingStateError,
+    CudaError,  // a HIP runtime failure on this build
+};
+
+class IReconstructionSession {
+public:
+    virtual ~IReconstructionSession() = default;
+    virtual bool initialize(const SSystemSpec& system) = 0;
+    virtual bool execute(const SExecutionRequest& request) = 0;
+    virtual bool isInitialized() const = 0;
+    virtual EApiErrorCode lastError() const = 0;
+    // valid until the next call on this session
+    virtual const char* lastErrorMessage() const = 0;
+    virtual void reset() = 0;
+    virtual void release() = 0;
+};
+
+struct ReconstructionSessionFactory {
+    static IReconstructionSession* create();
+    static void destroy(IReconstructionSession* session);
+};
+
+}  // namespace YK

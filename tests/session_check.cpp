@@ -1,0 +1,522 @@
+// Scenarios of the reference's Session-API tests run against the Session
+// facade of this library (include/yk_session.hpp, libyk_hip.so):
+//   dlltest/test_Task_dll.cpp:102-206, 427-505, 522-556 (validation and error
+//   codes, forward projection, FDK offline vs streaming, streaming state
+//   machine, reset / reuse) and dlltest/test_shepp_logan.cpp:32-158 (DLL FP
+//   then DLL FDK of a modified 3-D Shepp-Logan, 47-view streaming;
+//   cosine > 0.85, NRMSE < 0.65, streaming max difference < 1e-5).
+// Prints one "PASS <name>" / "FAIL <name>: <why>" line per scenario and
+// exits non-zero if any failed.  --host-only runs the scenarios that are
+// decided before any device work (they need no GPU).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "yk_session.hpp"
+
+namespace {
+
+using SessionPtr = std::unique_ptr<YK::IReconstructionSession, decltype(&YK::ReconstructionSessionFactory::destroy)>;
+
+SessionPtr make_session()
+{
+    return SessionPtr(YK::ReconstructionSessionFactory::create(), &YK::ReconstructionSessionFactory::destroy);
+}
+
+constexpr int kNu = 32, kNv = 24, kNx = 16, kNy = 16, kNz = 12;
+
+YK::SSystemSpec small_system(YK::EPipeline pipeline, int views)
+{
+    YK::SSystemSpec s{};
+    s.geometry.detector = YK::EDetectorKind::Flat;
+    s.geometry.trajectory = YK::ETrajectoryKind::Circular;
+    s.geometry.circular.total_views = views;
+    s.geometry.circular.views_per_turn = views;
+    s.geometry.circular.sid_mm = 500.f;
+    s.geometry.circular.sdd_mm = 1000.f;
+    s.geometry.flat_detector.channels = kNu;
+    s.geometry.flat_detector.rows = kNv;
+    s.geometry.flat_detector.channel_size_mm = 1.f;
+    s.geometry.flat_detector.row_size_mm = 1.f;
+    s.geometry.volume = { kNx, kNy, kNz, 1.f, 1.f, 1.f, make_float3(0.f, 0.f, 0.f) };
+    s.reconstruction.pipeline = pipeline;
+    return s;
+}
+
+// smooth, non-zero, reproducible projections
+std::vector<float> synthetic_projection(int views)
+{
+    std::vector<float> p(size_t(views) * kNu * kNv);
+    for (int i = 0; i < views; ++i)
+        for (int v = 0; v < kNv; ++v)
+            for (int u = 0; u < kNu; ++u)
+                p[(size_t(i) * kNv + v) * kNu + u] = 0.25f + 0.5f * float(u + 1) / kNu +
+                    0.125f * std::sin(6.2831853f * float(i + 1) / views) * float(v + 1) / kNv;
+    return p;
+}
+
+// device buffer with RAII
+struct DeviceBuffer {
+    float* p = nullptr;
+    size_t n = 0;
+    explicit DeviceBuffer(size_t count) : n(count)
+    {
+        if (hipMalloc(&p, count * sizeof(float)) != hipSuccess) p = nullptr;
+        else (void)hipMemset(p, 0, count * sizeof(float));
+    }
+    ~DeviceBuffer() { if (p) (void)hipFree(p); }
+    std::vector<float> download() const
+    {
+        std::vector<float> h(n);
+        if (p) (void)hipMemcpy(h.data(), p, n * sizeof(float), hipMemcpyDeviceToHost);
+        return h;
+    }
+    void upload(const std::vector<float>& h) { (void)hipMemcpy(p, h.data(), n * sizeof(float), hipMemcpyHostToDevice); }
+};
+
+YK::Buffer host(std::vector<float>& v) { return { v.data(), YK::EMemoryLocation::Host, v.size() }; }
+YK::Buffer dev(DeviceBuffer& b) { return { b.p, YK::EMemoryLocation::Device, b.n }; }
+
+double max_abs_diff(const std::vector<float>& a, const std::vector<float>& b)
+{
+    double m = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) m = std::max(m, std::abs(double(a[i]) - double(b[i])));
+    return m;
+}
+
+bool finite_nonzero(const std::vector<float>& v)
+{
+    double e = 0.0;
+    for (float x : v) {
+        if (!std::isfinite(x)) return false;
+        e += std::abs(double(x));
+    }
+    return e > 0.0;
+}
+
+struct Result {
+    bool ok;
+    std::string why;
+};
+
+Result expect_error(YK::IReconstructionSession& s, bool call_ok, YK::EApiErrorCode code)
+{
+    if (call_ok) return { false, "call succeeded" };
+    if (s.lastError() != code) return { false, "error code " + std::to_string(int(s.lastError())) + ": " + s.lastErrorMessage() };
+    if (!s.lastErrorMessage() || !s.lastErrorMessage()[0]) return { false, "empty error message" };
+    return { true, "" };
+}
+
+// ---------------------------------------------------------------------------
+// decided before device work
+// ---------------------------------------------------------------------------
+Result truncated_system_spec()
+{
+    auto s = make_session();
+    auto spec = small_system(YK::EPipeline::ForwardProjection, 2);
+    spec.struct_size = sizeof(std::uint32_t);
+    return expect_error(*s, s->initialize(spec), YK::EApiErrorCode::InvalidConfig);
+}
+
+Result unsupported_api_version()
+{
+    auto s = make_session();
+    auto spec = small_system(YK::EPipeline::FDK, 32);
+    spec.api_version = 999;
+    return expect_error(*s, s->initialize(spec), YK::EApiErrorCode::InvalidConfig);
+}
+
+Result enum_out_of_range()
+{
+    auto s = make_session();
+    auto spec = small_system(YK::EPipeline::FDK, 32);
+    spec.reconstruction.pipeline = static_cast<YK::EPipeline>(99);
+    Result r = expect_error(*s, s->initialize(spec), YK::EApiErrorCode::InvalidConfig);
+    if (!r.ok) return r;
+    spec = small_system(YK::EPipeline::ForwardProjection, 2);
+    spec.reconstruction.forward_projector = static_cast<YK::EForwardProjector>(-1);
+    return expect_error(*s, s->initialize(spec), YK::EApiErrorCode::InvalidConfig);
+}
+
+Result nan_pose_rejected()
+{
+    auto s = make_session();
+    auto spec = small_system(YK::EPipeline::ForwardProjection, 2);
+    spec.geometry.forward_projection_pose.enabled = true;
+    spec.geometry.forward_projection_pose.rotation_x_rad = NAN;
+    return expect_error(*s, s->initialize(spec), YK::EApiErrorCode::InvalidGeometry);
+}
+
+Result outside_this_path()
+{
+    auto s = make_session();
+    auto spec = small_system(YK::EPipeline::FDK, 32);
+    spec.geometry.trajectory = YK::ETrajectoryKind::Helical;
+    Result r = expect_error(*s, s->initialize(spec), YK::EApiErrorCode::UnsupportedCombination);
+    if (!r.ok) return r;
+    spec = small_system(YK::EPipeline::FDK, 32);
+    spec.geometry.detector = YK::EDetectorKind::Cylindrical;
+    r = expect_error(*s, s->initialize(spec), YK::EApiErrorCode::UnsupportedCombination);
+    if (!r.ok) return r;
+    spec = small_system(YK::EPipeline::SIRT, 32);
+    r = expect_error(*s, s->initialize(spec), YK::EApiErrorCode::UnsupportedCombination);
+    if (!r.ok) return r;
+    spec = small_system(YK::EPipeline::ForwardProjection, 2);
+    spec.reconstruction.forward_projector = YK::EForwardProjector::SeparableFootprint;
+    return expect_error(*s, s->initialize(spec), YK::EApiErrorCode::UnsupportedCombination);
+}
+
+Result invalid_scan_rejected()
+{
+    auto s = make_session();
+    auto spec = small_system(YK::EPipeline::FDK, 32);
+    spec.geometry.circular.sdd_mm = 100.f;  // detector inside the source circle
+    return expect_error(*s, s->initialize(spec), YK::EApiErrorCode::InvalidGeometry);
+}
+
+Result execute_before_initialize()
+{
+    auto s = make_session();
+    float x = 0.f;
+    YK::SExecutionRequest r{};
+    r.projection = { &x, YK::EMemoryLocation::Host, 1 };
+    r.volume = { &x, YK::EMemoryLocation::Host, 1 };
+    return expect_error(*s, s->execute(r), YK::EApiErrorCode::StreamingStateError);
+}
+
+// ---------------------------------------------------------------------------
+// device scenarios
+// ---------------------------------------------------------------------------
+Result forward_projection(YK::EForwardProjector kind)
+{
+    auto s = make_session();
+    auto spec = small_system(YK::EPipeline::ForwardProjection, 8);
+    spec.reconstruction.forward_projector = kind;
+    if (!s->initialize(spec)) return { false, s->lastErrorMessage() };
+    std::vector<float> vol(size_t(kNx) * kNy * kNz, 0.01f);
+    DeviceBuffer dvol(vol.size()), dproj(size_t(8) * kNu * kNv);
+    dvol.upload(vol);
+    YK::SExecutionRequest r{};
+    r.view_count = 8;
+    r.volume = dev(dvol);
+    r.projection = dev(dproj);
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    const auto p = dproj.download();
+    // the same call with host buffers gives the same projections
+    std::vector<float> hp(p.size(), -1.f);
+    r.volume = host(vol);
+    r.projection = host(hp);
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    if (!finite_nonzero(p)) return { false, "projection is zero or non-finite" };
+    if (max_abs_diff(p, hp) != 0.0) return { false, "host and device paths differ" };
+    return { true, "" };
+}
+
+Result execution_validation()
+{
+    auto s = make_session();
+    if (!s->initialize(small_system(YK::EPipeline::ForwardProjection, 2))) return { false, s->lastErrorMessage() };
+    YK::SExecutionRequest r{};
+    r.projection = { reinterpret_cast<float*>(1), YK::EMemoryLocation::Device, 1 };
+    r.volume = { reinterpret_cast<float*>(1), YK::EMemoryLocation::Device, 1 };
+    Result res = expect_error(*s, s->execute(r), YK::EApiErrorCode::InvalidBuffer);
+    if (!res.ok) return res;
+    r.projection.element_count = size_t(2) * kNu * kNv;
+    r.volume.element_count = size_t(kNx) * kNy * kNz;
+    r.view_offset = 1;
+    r.view_count = 1;
+    return expect_error(*s, s->execute(r), YK::EApiErrorCode::StreamingStateError);
+}
+
+Result fdk_offline_vs_streaming()
+{
+    constexpr int views = 32, first = 13;
+    const auto spec = small_system(YK::EPipeline::FDK, views);
+    auto offline = make_session(), streaming = make_session();
+    if (!offline->initialize(spec)) return { false, offline->lastErrorMessage() };
+    if (!streaming->initialize(spec)) return { false, streaming->lastErrorMessage() };
+    auto proj = synthetic_projection(views);
+    const size_t ve = size_t(kNu) * kNv, vn = size_t(kNx) * kNy * kNz;
+    DeviceBuffer a(vn), b(vn);
+    YK::SExecutionRequest r{};
+    r.view_count = views;
+    r.projection = host(proj);
+    r.volume = dev(a);
+    if (!offline->execute(r)) return { false, offline->lastErrorMessage() };
+    YK::SExecutionRequest r1{};
+    r1.view_offset = 0;
+    r1.view_count = first;
+    r1.clear_output = true;
+    r1.projection = { proj.data(), YK::EMemoryLocation::Host, size_t(first) * ve };
+    r1.volume = dev(b);
+    YK::SExecutionRequest r2 = r1;
+    r2.view_offset = first;
+    r2.view_count = views - first;
+    r2.clear_output = false;
+    r2.projection = { proj.data() + size_t(first) * ve, YK::EMemoryLocation::Host, size_t(views - first) * ve };
+    if (!streaming->execute(r1) || !streaming->execute(r2)) return { false, streaming->lastErrorMessage() };
+    const auto va = a.download(), vb = b.download();
+    if (!finite_nonzero(va)) return { false, "zero or non-finite volume" };
+    const double d = max_abs_diff(va, vb);
+    if (!(d <= 1e-5)) return { false, "offline / streaming max difference " + std::to_string(d) };
+    return { true, "" };
+}
+
+Result fdk_streaming_state_machine()
+{
+    constexpr int views = 32;
+    auto s = make_session();
+    if (!s->initialize(small_system(YK::EPipeline::FDK, views))) return { false, s->lastErrorMessage() };
+    auto proj = synthetic_projection(views);
+    const size_t ve = size_t(kNu) * kNv;
+    std::vector<float> vol(size_t(kNx) * kNy * kNz);
+    YK::SExecutionRequest r{};
+    r.projection = { proj.data(), YK::EMemoryLocation::Host, 8 * ve };
+    r.volume = host(vol);
+    r.view_offset = 4;  // skips views 0..3
+    r.view_count = 8;
+    Result res = expect_error(*s, s->execute(r), YK::EApiErrorCode::StreamingStateError);
+    if (!res.ok) return res;
+    r.view_offset = 0;
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    r.view_offset = 8;  // a later batch may not clear the output
+    r.clear_output = true;
+    res = expect_error(*s, s->execute(r), YK::EApiErrorCode::StreamingStateError);
+    if (!res.ok) return res;
+    r.clear_output = false;
+    r.projection.element_count = 8 * ve - 1;  // undersized projection
+    res = expect_error(*s, s->execute(r), YK::EApiErrorCode::InvalidBuffer);
+    if (!res.ok) return res;
+    r.projection.element_count = 8 * ve;
+    r.view_count = 40;  // beyond the prepared views
+    res = expect_error(*s, s->execute(r), YK::EApiErrorCode::InvalidConfig);
+    if (!res.ok) return res;
+    // a host volume is written once the last view has arrived
+    std::fill(vol.begin(), vol.end(), -7.f);
+    r.view_count = views - 8;
+    r.projection = { proj.data() + 8 * ve, YK::EMemoryLocation::Host, size_t(views - 8) * ve };
+    int progress_done = -1, progress_total = -1;
+    auto cb = [](void* user, int done, int total) {
+        int* p = static_cast<int*>(user);
+        p[0] = done;
+        p[1] = total;
+    };
+    int pr[2] = { -1, -1 };
+    r.progress_callback = cb;
+    r.progress_user_data = pr;
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    progress_done = pr[0];
+    progress_total = pr[1];
+    if (progress_done != views || progress_total != views) return { false, "progress callback not called with (32, 32)" };
+    if (!finite_nonzero(vol) || vol[0] == -7.f) return { false, "host volume not written at completion" };
+    // complete: a further batch is out of range
+    r.view_offset = views;
+    r.view_count = 1;
+    return expect_error(*s, s->execute(r), YK::EApiErrorCode::InvalidConfig);
+}
+
+Result session_reset_and_reuse()
+{
+    constexpr int views = 32;
+    auto s = make_session();
+    const auto spec = small_system(YK::EPipeline::FDK, views);
+    if (!s->initialize(spec)) return { false, s->lastErrorMessage() };
+    auto proj = synthetic_projection(views);
+    std::vector<float> v1(size_t(kNx) * kNy * kNz), v2(v1.size()), v3(v1.size());
+    YK::SExecutionRequest r{};
+    r.projection = host(proj);
+    r.volume = host(v1);
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    s->reset();
+    r.volume = host(v2);
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    s->release();
+    if (s->isInitialized()) return { false, "still initialized after release" };
+    if (!s->initialize(spec)) return { false, s->lastErrorMessage() };
+    r.volume = host(v3);
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    if (max_abs_diff(v1, v2) != 0.0 || max_abs_diff(v1, v3) != 0.0) return { false, "repeated runs differ" };
+    return { true, "" };
+}
+
+// modified 3-D Shepp-Logan (Kak & Slaney table), head half-extent 20 mm
+std::vector<float> shepp_logan(const YK::SVolumeGridSpec& g)
+{
+    struct E { double x, y, z, a, b, c, deg, val; };
+    const E table[] = {
+        { 0, 0, 0, .69, .92, .81, 0, 1 }, { 0, -.0184, 0, .6624, .874, .78, 0, -.8 },
+        { .22, 0, 0, .11, .31, .22, -18, -.2 }, { -.22, 0, 0, .16, .41, .28, 18, -.2 },
+        { 0, .35, 0, .21, .25, .41, 0, .1 }, { 0, .1, 0, .046, .046, .05, 0, .1 },
+        { 0, -.1, 0, .046, .046, .05, 0, .1 }, { -.08, -.605, 0, .046, .023, .05, 0, .1 },
+        { 0, -.605, 0, .023, .023, .02, 0, .1 }, { .06, -.605, 0, .023, .046, .02, 0, .1 },
+    };
+    std::vector<float> out(size_t(g.nx) * g.ny * g.nz);
+    for (int z = 0; z < g.nz; ++z)
+        for (int y = 0; y < g.ny; ++y)
+            for (int x = 0; x < g.nx; ++x) {
+                const double px = (x + .5 - .5 * g.nx) * g.voxel_x_mm / 20.0;
+                const double py = (y + .5 - .5 * g.ny) * g.voxel_y_mm / 20.0;
+                const double pz = (z + .5 - .5 * g.nz) * g.voxel_z_mm / 20.0;
+                double val = 0.0;
+                for (const E& e : table) {
+                    const double t = e.deg * 3.14159265358979323846 / 180.0;
+                    const double dx = px - e.x, dy = py - e.y, dz = pz - e.z;
+                    const double rx = dx * std::cos(t) - dy * std::sin(t), ry = dx * std::sin(t) + dy * std::cos(t);
+                    if (rx * rx / (e.a * e.a) + ry * ry / (e.b * e.b) + dz * dz / (e.c * e.c) <= 1.0) val += e.val;
+                }
+                out[(size_t(z) * g.ny + y) * g.nx + x] = float(std::max(val, 0.0) * 0.02);
+            }
+    return out;
+}
+
+Result shepp_logan_fp_then_fdk()
+{
+    YK::SSystemSpec spec{};
+    auto& g = spec.geometry;
+    g.circular.total_views = 360;
+    g.circular.views_per_turn = 360;
+    g.circular.sid_mm = 200.f;
+    g.circular.sdd_mm = 400.f;
+    g.flat_detector.channels = 128;
+    g.flat_detector.rows = 128;
+    g.flat_detector.channel_size_mm = 0.8f;
+    g.flat_detector.row_size_mm = 0.8f;
+    g.volume = { 64, 64, 64, 0.75f, 0.75f, 0.75f, make_float3(0.f, 0.f, 0.f) };
+    spec.reconstruction.pipeline = YK::EPipeline::FDK;
+    spec.reconstruction.fdk.filter.type = YK::EFdkFilter::RamLak;
+    auto truth = shepp_logan(g.volume);
+    const int views = 360;
+    const size_t ve = size_t(128) * 128;
+    std::vector<float> proj(ve * views), rec(truth.size()), streamed(truth.size());
+
+    auto fp = make_session();
+    auto fp_spec = spec;
+    fp_spec.reconstruction.pipeline = YK::EPipeline::ForwardProjection;
+    if (!fp->initialize(fp_spec)) return { false, fp->lastErrorMessage() };
+    YK::SExecutionRequest r{};
+    r.volume = host(truth);
+    r.projection = host(proj);
+    r.view_count = views;
+    if (!fp->execute(r)) return { false, fp->lastErrorMessage() };
+    fp.reset();
+
+    auto fdk = make_session();
+    if (!fdk->initialize(spec)) return { false, fdk->lastErrorMessage() };
+    r.volume = host(rec);
+    if (!fdk->execute(r)) return { false, fdk->lastErrorMessage() };
+    fdk->reset();
+    for (int off = 0; off < views; off += 47) {
+        const int n = std::min(47, views - off);
+        YK::SExecutionRequest b{};
+        b.projection = { proj.data() + size_t(off) * ve, YK::EMemoryLocation::Host, size_t(n) * ve };
+        b.volume = host(streamed);
+        b.view_offset = off;
+        b.view_count = n;
+        b.clear_output = off == 0;
+        if (!fdk->execute(b)) return { false, fdk->lastErrorMessage() };
+    }
+    double tt = 0, rr = 0, tr = 0, e2 = 0;
+    for (size_t i = 0; i < truth.size(); ++i) {
+        if (!std::isfinite(rec[i]) || !std::isfinite(streamed[i])) return { false, "non-finite reconstruction" };
+        const double t = truth[i], x = rec[i];
+        tt += t * t; rr += x * x; tr += t * x; e2 += (x - t) * (x - t);
+    }
+    const double cosine = tr / std::sqrt(std::max(tt * rr, 1e-30)), nrmse = std::sqrt(e2 / std::max(tt, 1e-30));
+    const double sd = max_abs_diff(rec, streamed);
+    std::printf("shepp-logan: cosine %.6f nrmse %.6f streaming max difference %.3g\n", cosine, nrmse, sd);
+    if (!(cosine > 0.85 && nrmse < 0.65 && sd < 1e-5)) return { false, "quality gates" };
+    return { true, "" };
+}
+
+// The pose rotates the volume about its centre: a quarter turn about Z maps
+// voxel (x, y) of the rotated volume to (y, N-1-x) of the stored one, so the
+// projections of the stored volume with the pose equal those of the
+// index-permuted volume without it.
+Result forward_projection_pose_quarter_turn()
+{
+    constexpr int n = 16, views = 12;
+    auto spec = small_system(YK::EPipeline::ForwardProjection, views);
+    spec.geometry.volume = { n, n, n, 1.f, 1.f, 1.f, make_float3(0.f, 0.f, 0.f) };
+    std::vector<float> v(size_t(n) * n * n), rot(v.size());
+    unsigned s = 12345u;
+    for (float& x : v) {
+        s = s * 1664525u + 1013904223u;
+        x = float((s >> 8) & 0xffff) / 65536.f;
+    }
+    for (int z = 0; z < n; ++z)
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x)
+                rot[(size_t(z) * n + y) * n + x] = v[(size_t(z) * n + (n - 1 - x)) * n + y];
+    std::vector<float> pa(size_t(views) * kNu * kNv), pb(pa.size());
+    auto a = make_session(), b = make_session();
+    auto posed = spec;
+    posed.geometry.forward_projection_pose.enabled = true;
+    posed.geometry.forward_projection_pose.rotation_z_rad = 1.5707963267948966f;
+    if (!a->initialize(posed)) return { false, a->lastErrorMessage() };
+    if (!b->initialize(spec)) return { false, b->lastErrorMessage() };
+    YK::SExecutionRequest r{};
+    r.volume = host(v);
+    r.projection = host(pa);
+    if (!a->execute(r)) return { false, a->lastErrorMessage() };
+    r.volume = host(rot);
+    r.projection = host(pb);
+    if (!b->execute(r)) return { false, b->lastErrorMessage() };
+    double num = 0, den = 0;
+    for (size_t i = 0; i < pa.size(); ++i) {
+        num += (double(pa[i]) - pb[i]) * (double(pa[i]) - pb[i]);
+        den += double(pb[i]) * pb[i];
+    }
+    const double rel = std::sqrt(num / std::max(den, 1e-30));
+    if (!(den > 0 && rel < 1e-4)) return { false, "posed vs permuted projections differ: rel " + std::to_string(rel) };
+    return { true, "" };
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    const bool host_only = argc > 1 && std::strcmp(argv[1], "--host-only") == 0;
+    struct Case {
+        const char* name;
+        std::function<Result()> run;
+        bool device;
+    };
+    const std::vector<Case> cases = {
+        { "truncated_system_spec", truncated_system_spec, false },
+        { "unsupported_api_version", unsupported_api_version, false },
+        { "enum_out_of_range", enum_out_of_range, false },
+        { "nan_pose_rejected", nan_pose_rejected, false },
+        { "outside_this_path", outside_this_path, false },
+        { "invalid_scan_rejected", invalid_scan_rejected, false },
+        { "execute_before_initialize", execute_before_initialize, false },
+        { "forward_projection_siddon", [] { return forward_projection(YK::EForwardProjector::Siddon); }, true },
+        { "forward_projection_joseph", [] { return forward_projection(YK::EForwardProjector::Joseph); }, true },
+        { "forward_projection_automatic", [] { return forward_projection(YK::EForwardProjector::Automatic); }, true },
+        { "execution_validation", execution_validation, true },
+        { "fdk_offline_vs_streaming", fdk_offline_vs_streaming, true },
+        { "fdk_streaming_state_machine", fdk_streaming_state_machine, true },
+        { "session_reset_and_reuse", session_reset_and_reuse, true },
+        { "forward_projection_pose_quarter_turn", forward_projection_pose_quarter_turn, true },
+        { "shepp_logan_fp_then_fdk", shepp_logan_fp_then_fdk, true },
+    };
+    int failed = 0;
+    for (const Case& c : cases) {
+        if (host_only && c.device) continue;
+        std::fflush(stdout);
+        const Result r = c.run();
+        if (r.ok) std::printf("PASS %s\n", c.name);
+        else {
+            std::printf("FAIL %s: %s\n", c.name, r.why.c_str());
+            ++failed;
+        }
+        std::fflush(stdout);
+    }
+    return failed == 0 ? 0 : 1;
+}

@@ -32,6 +32,28 @@ def test_library_exports_every_declared_symbol():
     assert lib.yk_abi_version() == 1
 
 
+def test_library_exports_the_session_factory():
+    """include/yk_session.hpp: ReconstructionSessionFactory::create / destroy
+    (the reference's YkReconstructionApi.hpp:95-98) are exported C++ symbols."""
+    lib = C.lib()
+    for name in ("_ZN2YK28ReconstructionSessionFactory6createEv",
+                 "_ZN2YK28ReconstructionSessionFactory7destroyEPNS_22IReconstructionSessionE"):
+        assert hasattr(lib, name), name
+
+
+def test_session_facade_validation_without_device():
+    """Session-API scenarios decided before any device work (struct_size,
+    api_version, enum ranges, pose, unsupported combinations, scan validity,
+    execute before initialize): tests/session_check.cpp --host-only."""
+    import subprocess
+    from fdk_cuda_impl_amd.build import SESSION_CHECK
+    assert os.path.exists(SESSION_CHECK), "build() compiles the Session check program"
+    res = subprocess.run([SESSION_CHECK, "--host-only"], capture_output=True, text=True, timeout=60)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert res.stdout.count("PASS") == 7
+
+
 def test_struct_sizes_match_reference_pod_layout():
     # SConeProjGeomVec is 5 x float4 (YkProjectionGeometry.hpp:12-18)
     assert ctypes.sizeof(C.View) == 80
