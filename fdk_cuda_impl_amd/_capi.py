@@ -84,6 +84,18 @@ class FdkDesc(ctypes.Structure):
                 ("kahan", ctypes.c_int), ("batch_views", ctypes.c_int)]
 
 
+class AlgebraicDesc(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("method", ctypes.c_int),
+                ("weight_model", ctypes.c_int), ("subset_order", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("subsets", ctypes.c_int),
+                ("relaxation", ctypes.c_float), ("relaxation_reduction", ctypes.c_float),
+                ("epsilon", ctypes.c_float), ("use_min", ctypes.c_int), ("min_value", ctypes.c_float),
+                ("use_max", ctypes.c_int), ("max_value", ctypes.c_float),
+                ("forward_kind", ctypes.c_int), ("back_kind", ctypes.c_int),
+                ("tv_strength", ctypes.c_float), ("tv_iterations", ctypes.c_int),
+                ("tv_epsilon", ctypes.c_float), ("tv_strength_reduction", ctypes.c_float)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _F = ctypes.c_float
@@ -129,6 +141,7 @@ SIGNATURES = {
     "yk_backproject": (_I, [_P, _I, _P, _P, _I, _I, _I]),
     "yk_cgls": (_I, [_P, _I, _I, _P, _P, _I, _DP]),
     "yk_sart": (_I, [_P, _I, _I, _P, _P, _I, _F, _I]),
+    "yk_algebraic": (_I, [_P, ctypes.POINTER(AlgebraicDesc), _P, _P]),
     "yk_tv_gradient": (_I, [_P, ctypes.POINTER(VolumeGrid), _P, _F, _P, _DP]),
     "yk_dot": (_I, [_P, _P, _P, _SZ, _DP]),
     "yk_phantom_ellipsoids": (_I, [_P, _FP, _I, ctypes.POINTER(VolumeGrid), _I, _P]),

@@ -403,6 +403,9 @@ def build_views_from_system(system: C.System) -> np.ndarray:
 FORWARD = {"joseph": C.FP_JOSEPH, "siddon": C.FP_SIDDON}
 BACK = {"joseph": C.BP_JOSEPH_ADJOINT, "siddon": C.BP_SIDDON_ADJOINT,
         "fdk": C.BP_FDK, "fdk_pseudo_matched": C.BP_FDK_PSEUDO_MATCHED}
+ALGEBRAIC = {"sirt": 0, "sart": 1, "ossart": 2}  # yk_algebraic_method
+WEIGHT_MODELS = {"detailed": 0, "tigre": 1}     # yk_weight_model
+SUBSET_ORDERS = {"sequential": 0, "golden": 1}  # yk_subset_order
 
 
 class Projector:
@@ -465,6 +468,53 @@ class Projector:
             x = torch.zeros(self.volume_shape, dtype=torch.float32, device=b.device)
         C.call("yk_sart", self.handle, FORWARD[forward], BACK[back], _ptr(b), _ptr(x), iterations,
                relaxation, 1 if nonnegative else 0)
+        return x
+
+    def algebraic(self, b, method="sart", x=None, iterations=10, relaxation=1.0, subsets=1,
+                  weights=None, order=None, relaxation_reduction=1.0, epsilon=1e-6, min_value=None,
+                  max_value=None, forward="joseph", back="joseph", tv_strength=0.0, tv_iterations=0,
+                  tv_epsilon=1e-8, tv_strength_reduction=1.0):
+        """SIRT / SART / OS-SART on the device, updating x (zeros when None) in place.
+
+        ``weights`` is "detailed" (exact per-subset row and column sums) or
+        "tigre" (TIGRE-style approximate row weights over contiguous view
+        blocks); the defaults follow the reference's public entry points:
+        SART uses "tigre", SIRT and OS-SART "detailed", OS-SART with the
+        golden-ratio subset order (YkAlgebraicReconstructorEx.hpp:15-180,
+        YkAlgebraicBackends.hpp:32-310, 597-800).
+        """
+        torch = _torch()
+        if method not in ALGEBRAIC:
+            raise ValueError(f"method must be one of {sorted(ALGEBRAIC)}")
+        _check_cuda(b, torch.float32, (self.n_views, self.nv, self.nu), "projection")
+        if x is None:
+            x = torch.zeros(self.volume_shape, dtype=torch.float32, device=b.device)
+        _check_cuda(x, torch.float32, self.volume_shape, "volume")
+        if weights is None:
+            weights = "tigre" if method == "sart" else "detailed"
+        if order is None:
+            order = "golden" if method == "ossart" else "sequential"
+        d = C.AlgebraicDesc()
+        d.struct_size = ctypes.sizeof(C.AlgebraicDesc)
+        d.method = ALGEBRAIC[method]
+        d.weight_model = WEIGHT_MODELS[weights]
+        d.subset_order = SUBSET_ORDERS[order]
+        d.iterations = iterations
+        d.subsets = subsets
+        d.relaxation = relaxation
+        d.relaxation_reduction = relaxation_reduction
+        d.epsilon = epsilon
+        d.use_min = 0 if min_value is None else 1
+        d.min_value = 0.0 if min_value is None else min_value
+        d.use_max = 0 if max_value is None else 1
+        d.max_value = 0.0 if max_value is None else max_value
+        d.forward_kind = FORWARD[forward]
+        d.back_kind = BACK[back]
+        d.tv_strength = tv_strength
+        d.tv_iterations = tv_iterations
+        d.tv_epsilon = tv_epsilon
+        d.tv_strength_reduction = tv_strength_reduction
+        C.call("yk_algebraic", self.handle, ctypes.byref(d), _ptr(b), _ptr(x))
         return x
 
     def close(self):

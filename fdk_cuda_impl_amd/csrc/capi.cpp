@@ -449,8 +449,8 @@ yk_status yk_cgls(yk_projector* p, int fk, int bk, const float* d_b, float* d_x,
     });
 }
 
-// SART: view-by-view x += lambda A_i^T((b_i - A_i x) / A_i 1) / A_i^T 1
-// (AlgebraicReconstructorEx per-view order,
+// SART: view-by-view updates with exact per-view weights, the DETAILED
+// SART schedule of yk_algebraic (AlgebraicReconstructorEx per-view order,
 // src/Reconstruction/Iterative/Flat/YkAlgebraicReconstructorEx.hpp:137-180)
 yk_status yk_sart(yk_projector* p, int fk, int bk, const float* d_b, float* d_x, int iterations,
     float relaxation, int nonnegative)
@@ -460,22 +460,33 @@ yk_status yk_sart(yk_projector* p, int fk, int bk, const float* d_b, float* d_x,
         require(d_b && d_x, YK_ERR_INVALID_BUFFER, "null device buffer");
         require(iterations >= 0 && std::isfinite(relaxation) && relaxation > 0.f, YK_ERR_INVALID_CONFIG,
             "invalid SART parameters");
-        p->ctx->activate();
-        hipStream_t s = p->ctx->stream;
-        const size_t nvw = size_t(p->nu) * p->nv;
-        const size_t nx = size_t(p->grid.nx) * p->grid.ny * p->grid.nz;
-        DeviceArray<float> ones_v(nx), ones_p(nvw), row(nvw), res(nvw), col(nx), upd(nx);
-        launch_fill(ones_v.get(), 1.f, nx, s);
-        launch_fill(ones_p.get(), 1.f, nvw, s);
-        for (int it = 0; it < iterations; ++it)
-            for (int i = 0; i < p->n_views; ++i) {
-                p->project(fk, ones_v.get(), row.get(), i, 1, 0);
-                p->project(fk, d_x, res.get(), i, 1, 0);
-                sart_residual(res.get(), d_b + i * nvw, row.get(), nvw, s);
-                p->backproject(bk, res.get(), upd.get(), i, 1, 0);
-                p->backproject(bk, ones_p.get(), col.get(), i, 1, 0);
-                sart_update(d_x, upd.get(), col.get(), relaxation, nonnegative, nx, s);
-            }
+        if (iterations == 0) return;
+        yk_algebraic_desc d{};
+        d.struct_size = sizeof(d);
+        d.method = YK_ALG_SART;
+        d.weight_model = YK_WEIGHTS_DETAILED;
+        d.iterations = iterations;
+        d.relaxation = relaxation;
+        d.relaxation_reduction = 1.f;
+        d.epsilon = 1e-6f;
+        d.use_min = nonnegative ? 1 : 0;
+        d.min_value = 0.f;
+        d.forward_kind = fk;
+        d.back_kind = bk;
+        d.tv_strength_reduction = 1.f;
+        run_algebraic(p, d, d_b, d_x);
+        p->ctx->finish();
+    });
+}
+
+yk_status yk_algebraic(yk_projector* p, const yk_algebraic_desc* desc, const float* d_b, float* d_x)
+{
+    return guard([&] {
+        require(p != nullptr && desc != nullptr, YK_ERR_INVALID_CONFIG, "null projector or descriptor");
+        require(desc->struct_size >= sizeof(yk_algebraic_desc), YK_ERR_INVALID_CONFIG,
+            "yk_algebraic_desc is truncated or from an older layout");
+        require(d_b && d_x, YK_ERR_INVALID_BUFFER, "null device buffer");
+        run_algebraic(p, *desc, d_b, d_x);
         p->ctx->finish();
     });
 }

@@ -81,6 +81,10 @@ struct yk_projector {
     std::vector<char> siddon_separable;  // per view: detV along Z only
     yk::DeviceArray<float2> d_sid_col;   // separable Siddon transpose ray table, one view chunk
     yk::DeviceArray<float2> d_sid_ray;
+    // projector whose scratch arrays (volume copies, ray tables) this one uses;
+    // null: its own.  Operators of one volume grid that run one after another
+    // on one stream can share them.
+    yk_projector* scratch_owner = nullptr;
     yk::DeviceArray<float> d_vol_t;
     yk::DeviceArray<float> d_vol_zx;   // Z-fastest volume copies of the separable Joseph projection
     yk::DeviceArray<float> d_vol_zy;
@@ -88,3 +92,6 @@ struct yk_projector {
     yk::DeviceArray<yk::OpBpViewDev> d_opviews;  // operator-form FDK backprojection records
     std::vector<double> op_scale;      // per first view: vx vy vz / (du dv)
 };
+
+// unified SIRT / SART / OS-SART (algebraic.cpp); throws through fail()
+void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b, float* d_x);

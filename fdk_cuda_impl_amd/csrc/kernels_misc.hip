@@ -61,24 +61,51 @@ __global__ void fill_kernel(float* __restrict__ y, float v, size_t n)
         y[i] = v;
 }
 
-__global__ void sart_residual_kernel(float* __restrict__ res, const float* __restrict__ b,
-    const float* __restrict__ row, size_t n)
+// Algebraic (SIRT / SART / OS-SART) element steps
+// (src/Reconstruction/Iterative/Common/kernels/YkIterKernels.cu:202-256, 299-318):
+//   mode 0: res = row > eps ? (b - res) / row : 0     (residual, divide_launch)
+//   mode 1: res = (b - res) * row                     (residual, multiply_launch)
+__global__ void algebraic_residual_kernel(float* __restrict__ res, const float* __restrict__ b,
+    const float* __restrict__ row, float eps, int mode, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float r = b[i] - res[i];
         const float w = row[i];
-        res[i] = w > 1e-6f ? (b[i] - res[i]) / w : 0.f;
+        res[i] = mode == 0 ? (w > eps ? r / w : 0.f) : r * w;
     }
 }
 
-__global__ void sart_update_kernel(float* __restrict__ x, const float* __restrict__ upd,
-    const float* __restrict__ col, float lambda, int nonnegative, size_t n)
+// x += lambda bp / (col + eps), then the optional bounds (update_launch,
+// clamp_min_launch, clamp_max_launch)
+__global__ void algebraic_update_kernel(float* __restrict__ x, const float* __restrict__ bp,
+    const float* __restrict__ col, float lambda, float eps, int use_min, float lo, int use_max, float hi,
+    size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const float c = col[i];
-        float v = x[i];
-        if (c > 1e-6f) v += lambda * upd[i] / c;
-        if (nonnegative) v = fmaxf(v, 0.f);
+        float v = x[i] + lambda * bp[i] / (col[i] + eps);
+        if (use_min && v < lo) v = lo;
+        if (use_max && v > hi) v = hi;
         x[i] = v;
+    }
+}
+
+__global__ void bounds_kernel(float* __restrict__ x, int use_min, float lo, int use_max, float hi, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float v = x[i];
+        if (use_min && v < lo) v = lo;
+        if (use_max && v > hi) v = hi;
+        x[i] = v;
+    }
+}
+
+// mode 0: x <= t -> 1e30 (threshold_inf_launch); mode 1: additionally x = 1 / x
+__global__ void weight_prepare_kernel(float* __restrict__ x, float t, int reciprocal, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float v = x[i];
+        if (v <= t) v = 1e30f;
+        x[i] = reciprocal ? 1.f / v : v;
     }
 }
 
@@ -194,17 +221,30 @@ void launch_fill(float* y, float v, size_t n, hipStream_t s)
     YK_HIP(hipGetLastError());
 }
 
-void sart_residual(float* res, const float* b, const float* row, size_t n, hipStream_t s)
+void algebraic_residual(float* res, const float* b, const float* row, float eps, int mode, size_t n,
+    hipStream_t s)
 {
-    hipLaunchKernelGGL(sart_residual_kernel, dim3(grid_for(n)), dim3(256), 0, s, res, b, row, n);
+    hipLaunchKernelGGL(algebraic_residual_kernel, dim3(grid_for(n)), dim3(256), 0, s, res, b, row, eps, mode, n);
     YK_HIP(hipGetLastError());
 }
 
-void sart_update(float* x, const float* upd, const float* col, float lambda, int nonnegative, size_t n,
-    hipStream_t s)
+void algebraic_update(float* x, const float* bp, const float* col, float lambda, float eps, int use_min,
+    float lo, int use_max, float hi, size_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(sart_update_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, upd, col, lambda,
-        nonnegative, n);
+    hipLaunchKernelGGL(algebraic_update_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, bp, col, lambda, eps,
+        use_min, lo, use_max, hi, n);
+    YK_HIP(hipGetLastError());
+}
+
+void launch_bounds_clamp(float* x, int use_min, float lo, int use_max, float hi, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(bounds_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, use_min, lo, use_max, hi, n);
+    YK_HIP(hipGetLastError());
+}
+
+void weight_prepare(float* x, float threshold, int reciprocal, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(weight_prepare_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, threshold, reciprocal, n);
     YK_HIP(hipGetLastError());
 }
 

@@ -107,6 +107,7 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
     if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
     if (!d_volume || !d_proj) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
     hipStream_t s = ctx->stream;
+    yk_projector& w = scratch_owner ? *scratch_owner : *this;  // scratch arrays
     bool all_sep = true, any_x = false, any_y = false;
     for (int i = first; i < first + count; ++i) {
         all_sep = all_sep && separable[i];
@@ -117,16 +118,16 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
     if (kind == YK_FP_JOSEPH && all_sep) {
         const size_t nvox = size_t(grid.nx) * grid.ny * grid.nz;
         if (any_x) {
-            d_vol_zx.resize(nvox);
-            launch_transpose_zfast(d_volume, d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
+            w.d_vol_zx.resize(nvox);
+            launch_transpose_zfast(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
         }
         if (any_y) {
-            d_vol_zy.resize(nvox);
-            launch_transpose_zfast(d_volume, d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, s);
+            w.d_vol_zy.resize(nvox);
+            launch_transpose_zfast(d_volume, w.d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, s);
         }
         JosephSepFpLaunch p{};
-        p.vol_zx = any_x ? d_vol_zx.get() : nullptr;
-        p.vol_zy = any_y ? d_vol_zy.get() : nullptr;
+        p.vol_zx = any_x ? w.d_vol_zx.get() : nullptr;
+        p.vol_zy = any_y ? w.d_vol_zy.get() : nullptr;
         p.proj = d_proj;
         p.views = d_jviews.get() + first;
         p.vol = vol;
@@ -139,12 +140,12 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
         for (int i = first; i < first + count; ++i)
             if (main_axis(views[i], nu, nv) == 0) need_t = true;
         if (need_t) {
-            d_vol_t.resize(size_t(grid.nx) * grid.ny * grid.nz);
-            launch_transpose_xy(d_volume, d_vol_t.get(), grid.nx, grid.ny, grid.nz, s);
+            w.d_vol_t.resize(size_t(grid.nx) * grid.ny * grid.nz);
+            launch_transpose_xy(d_volume, w.d_vol_t.get(), grid.nx, grid.ny, grid.nz, s);
         }
         JosephLaunch p{};
         p.volume = d_volume;
-        p.volume_t = need_t ? d_vol_t.get() : nullptr;
+        p.volume_t = need_t ? w.d_vol_t.get() : nullptr;
         p.proj = d_proj;
         p.views = d_jviews.get() + first;
         p.vol = vol;
@@ -154,10 +155,10 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
     } else if (kind == YK_FP_SIDDON && std::all_of(siddon_separable.begin() + first,
                                                    siddon_separable.begin() + first + count,
                                                    [](char c) { return c != 0; })) {
-        d_vol_zx.resize(size_t(grid.nx) * grid.ny * grid.nz);
-        launch_transpose_zfast(d_volume, d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
+        w.d_vol_zx.resize(size_t(grid.nx) * grid.ny * grid.nz);
+        launch_transpose_zfast(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
         SiddonSepFpLaunch p{};
-        p.vol_zx = d_vol_zx.get();
+        p.vol_zx = w.d_vol_zx.get();
         p.proj = d_proj;
         p.views = d_sviews.get() + first;
         p.vol = vol;
@@ -184,6 +185,7 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
     if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
     if (!d_volume || !d_proj) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
     hipStream_t s = ctx->stream;
+    yk_projector& w = scratch_owner ? *scratch_owner : *this;  // scratch arrays
     bool all_sep = true;
     for (int i = first; i < first + count; ++i) all_sep = all_sep && separable[i];
     if (kind == YK_BP_JOSEPH_ADJOINT && all_sep) {
@@ -192,18 +194,18 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
         const size_t table = size_t(nu) * (nv + kSepPad);
         const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(count),
             (size_t(1) << 30) / (table * (sizeof(float2) + sizeof(float)) + size_t(nu) * sizeof(float2)))));
-        d_sep_ab1.resize(std::max(d_sep_ab1.size(), size_t(nu) * chunk));
-        d_sep_ab2.resize(std::max(d_sep_ab2.size(), table * chunk));
-        d_sep_pl.resize(std::max(d_sep_pl.size(), table * chunk));
+        w.d_sep_ab1.resize(std::max(w.d_sep_ab1.size(), size_t(nu) * chunk));
+        w.d_sep_ab2.resize(std::max(w.d_sep_ab2.size(), table * chunk));
+        w.d_sep_pl.resize(std::max(w.d_sep_pl.size(), table * chunk));
         for (int c0 = 0; c0 < count; c0 += chunk) {
             JosephSepLaunch p{};
             p.proj = d_proj + per_view * c0;
             p.volume = d_volume;
             p.views = d_jviews.get() + first + c0;
             p.coeff32 = d_c32.get() + size_t(first + c0) * 12;
-            p.ab1 = d_sep_ab1.get();
-            p.ab2 = d_sep_ab2.get();
-            p.pl = d_sep_pl.get();
+            p.ab1 = w.d_sep_ab1.get();
+            p.ab2 = w.d_sep_ab2.get();
+            p.pl = w.d_sep_pl.get();
             p.vol = vol;
             p.nu = nu; p.nv = nv; p.count = std::min(chunk, count - c0);
             p.accumulate = c0 > 0 ? 1 : accumulate;
@@ -214,16 +216,16 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
         const size_t per_view = size_t(nu) * nv;
         const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(count),
             (size_t(1) << 30) / (per_view * (sizeof(float4) + sizeof(float))))));
-        d_ray_ab.resize(std::max(d_ray_ab.size(), per_view * chunk));
-        d_ray_len.resize(std::max(d_ray_len.size(), per_view * chunk));
+        w.d_ray_ab.resize(std::max(w.d_ray_ab.size(), per_view * chunk));
+        w.d_ray_len.resize(std::max(w.d_ray_len.size(), per_view * chunk));
         for (int c0 = 0; c0 < count; c0 += chunk) {
             JosephAdjointLaunch p{};
             p.proj = d_proj + per_view * c0;
             p.volume = d_volume;
             p.views = d_jviews.get() + first + c0;
             p.coeff32 = d_c32.get() + size_t(first + c0) * 12;
-            p.ray_ab = d_ray_ab.get();
-            p.ray_len = d_ray_len.get();
+            p.ray_ab = w.d_ray_ab.get();
+            p.ray_len = w.d_ray_len.get();
             p.vol = vol;
             p.nu = nu; p.nv = nv; p.count = std::min(chunk, count - c0);
             p.accumulate = c0 > 0 ? 1 : accumulate;
@@ -237,16 +239,16 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
         const size_t table = size_t(nu) * (nv + kSepPad);
         const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(count),
             (size_t(1) << 30) / ((table + nu) * sizeof(float2)))));
-        d_sid_col.resize(std::max(d_sid_col.size(), size_t(nu) * chunk));
-        d_sid_ray.resize(std::max(d_sid_ray.size(), table * chunk));
+        w.d_sid_col.resize(std::max(w.d_sid_col.size(), size_t(nu) * chunk));
+        w.d_sid_ray.resize(std::max(w.d_sid_ray.size(), table * chunk));
         for (int c0 = 0; c0 < count; c0 += chunk) {
             SiddonSepLaunch p{};
             p.proj = d_proj + per_view * c0;
             p.volume = d_volume;
             p.views = d_sviews.get() + first + c0;
             p.coeff32 = d_c32.get() + size_t(first + c0) * 12;
-            p.col_xy = d_sid_col.get();
-            p.ray_wz = d_sid_ray.get();
+            p.col_xy = w.d_sid_col.get();
+            p.ray_wz = w.d_sid_ray.get();
             p.vol = vol;
             p.nu = nu; p.nv = nv; p.count = std::min(chunk, count - c0);
             p.accumulate = c0 > 0 ? 1 : accumulate;

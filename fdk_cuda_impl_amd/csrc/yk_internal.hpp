@@ -382,11 +382,15 @@ void launch_axpy(float* y, const float* x, float a, size_t n, hipStream_t s);   
 void launch_xpby(float* y, const float* x, float b, size_t n, hipStream_t s);        // y = x + b y
 void launch_scale_copy(float* y, const float* x, float a, size_t n, hipStream_t s);  // y = a x
 void launch_fill(float* y, float v, size_t n, hipStream_t s);
-// res = (b - res) / row where row > eps, else 0
-void sart_residual(float* res, const float* b, const float* row, size_t n, hipStream_t s);
-// x += lambda upd / col where col > eps; optional clamp at 0
-void sart_update(float* x, const float* upd, const float* col, float lambda, int nonnegative, size_t n,
+// algebraic steps: mode 0 res = row > eps ? (b - res) / row : 0, mode 1 res = (b - res) row
+void algebraic_residual(float* res, const float* b, const float* row, float eps, int mode, size_t n,
     hipStream_t s);
+// x += lambda bp / (col + eps), then optional bounds
+void algebraic_update(float* x, const float* bp, const float* col, float lambda, float eps, int use_min,
+    float lo, int use_max, float hi, size_t n, hipStream_t s);
+void launch_bounds_clamp(float* x, int use_min, float lo, int use_max, float hi, size_t n, hipStream_t s);
+// x <= threshold -> 1e30, then optionally x = 1 / x
+void weight_prepare(float* x, float threshold, int reciprocal, size_t n, hipStream_t s);
 void launch_tv_gradient(const float* x, float* grad, double* partials, int n_partials,
     const VolumeGeom& g, float eps, hipStream_t s);
 void launch_phantom(const float* ellipsoids, int n, const VolumeGeom& g, int clamp,

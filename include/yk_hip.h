@@ -291,6 +291,50 @@ yk_status yk_cgls(yk_projector* projector, int forward_kind, int back_kind,
 yk_status yk_sart(yk_projector* projector, int forward_kind, int back_kind,
     const float* d_b, float* d_x, int iterations, float relaxation,
     int nonnegative);
+/* Unified algebraic reconstruction (AlgebraicReconstructorEx,
+ * src/Reconstruction/Iterative/Flat/YkAlgebraicReconstructorEx.hpp:15-180):
+ *   x <- P[x + lambda C_s^-1 B_s R_s^-1 (b_s - A_s x)] over subsets s, then
+ *   after each outer iteration smoothed-TV steps and the bounds again.
+ * Subsets: SIRT one (all views), SART one per view, OS-SART `subsets`.
+ * Weight models:
+ *   DETAILED (AlgebraicDetailedWeightBackend, YkAlgebraicBackends.hpp:597-800;
+ *   SirtBatchBackend, YkSirtBatchBackend.hpp:11-80): interleaved subsets
+ *   (views s, s+n, ...), R = A_s 1 with R > eps ? r / R : 0, C = B_s 1
+ *   (SIRT: C <= 0 -> 1e30), sequential or golden-ratio subset order, lambda
+ *   reduced by reduction^(1/subsets) after every subset update;
+ *   TIGRE (AlgebraicTigreBackend, YkAlgebraicBackends.hpp:32-310): contiguous
+ *   blocks of ceil(n/subsets) views in order, R^-1 from the projection of a
+ *   2 x 2 x 2 support volume (1.1 x the XY extent, max(detector, volume) Z
+ *   extent; <= half the smallest voxel -> 0), C = B_s 1 with C <= 0 -> 1e30,
+ *   lambda reduced once per outer iteration.
+ * The update is x += lambda bp / (C + eps).  d_x holds the initial volume and
+ * receives the result; d_b is the whole sinogram [view][v][u]. */
+typedef enum yk_algebraic_method { YK_ALG_SIRT = 0, YK_ALG_SART = 1, YK_ALG_OSSART = 2 } yk_algebraic_method;
+typedef enum yk_weight_model { YK_WEIGHTS_DETAILED = 0, YK_WEIGHTS_TIGRE = 1 } yk_weight_model;
+typedef enum yk_subset_order { YK_ORDER_SEQUENTIAL = 0, YK_ORDER_GOLDEN = 1 } yk_subset_order;
+typedef struct yk_algebraic_desc {
+    uint32_t struct_size;        /* sizeof(yk_algebraic_desc) */
+    int method;                  /* yk_algebraic_method */
+    int weight_model;            /* yk_weight_model */
+    int subset_order;            /* yk_subset_order (DETAILED OS-SART only) */
+    int iterations;              /* outer iterations, >= 1 (every subset once each) */
+    int subsets;                 /* OS-SART subset count in [1, views] */
+    float relaxation;            /* lambda */
+    float relaxation_reduction;  /* per outer iteration */
+    float epsilon;               /* weight guard, > 0 (1e-6 in the reference) */
+    int use_min;
+    float min_value;
+    int use_max;
+    float max_value;
+    int forward_kind;            /* yk_forward_kind */
+    int back_kind;               /* yk_back_kind */
+    float tv_strength;           /* smoothed-TV step x -= strength grad (0: off) */
+    int tv_iterations;           /* TV steps after each outer iteration */
+    float tv_epsilon;
+    float tv_strength_reduction; /* strength *= this after each outer iteration */
+} yk_algebraic_desc;
+yk_status yk_algebraic(yk_projector* projector, const yk_algebraic_desc* desc,
+    const float* d_b, float* d_x);
 /* smoothed TV value and gradient (YkTvRegularizationKernels.cu:17-73) */
 yk_status yk_tv_gradient(yk_ctx* ctx, const yk_volume_grid* volume, const float* d_x,
     float epsilon, float* d_gradient, double* value);

@@ -334,3 +334,143 @@ def fdk(views, proj, vol: Volume, filt="ramlak", scan_range=None, cutoff=0.5, ga
     su = [start_u(nu, N, d[i, D_OFFU]) for i in range(n)]
     f = filter_rows(q, w, N, su, method=method)
     return bp_fdk(c, f, vol, voxel_index)
+
+
+# --- algebraic reconstruction (SIRT / SART / OS-SART) ----------------------
+
+ALGEBRAIC_METHODS = ("sirt", "sart", "ossart")
+
+
+def golden_subset_order(n):
+    """Golden-ratio subset visiting order: step round(n / phi) made coprime
+    with n by incrementing (falling back to 1), then 0, step, 2 step, ... mod n
+    (AlgebraicDetailedWeightBackend::init,
+    src/Reconstruction/Iterative/Flat/YkAlgebraicBackends.hpp:669-684)."""
+    if n <= 1:
+        return list(range(n))
+    step = max(1, int(np.floor(n / 1.6180339887498949 + 0.5)))
+    while np.gcd(step, n) != 1:
+        step += 1
+        if step >= n:
+            step = 1
+            break
+    return [(i * step) % n for i in range(n)]
+
+
+def _operators(kind):
+    fwd = {"joseph": fp_joseph, "siddon": fp_siddon}[kind[0]]
+    back = {"joseph": bp_joseph_adjoint, "siddon": bp_siddon_adjoint}[kind[1]]
+    return fwd, back
+
+
+def tigre_support(views, nv, vol: Volume) -> Volume:
+    """2 x 2 x 2 support grid of the TIGRE row weights: voxel (1.1 Nx vx / 2,
+    1.1 Ny vy / 2, max(Nv dv, Nz vz) / 2), same centre
+    (AlgebraicTigreBackend::init, YkAlgebraicBackends.hpp:125-134)."""
+    dv = float(np.linalg.norm(_d(views)[0, 9:12]))
+    return volume(2, 2, 2, vol.nx * vol.vx * 1.1 / 2, vol.ny * vol.vy * 1.1 / 2,
+                  max(nv * dv, vol.nz * vol.vz) / 2, (vol.cx, vol.cy, vol.cz))
+
+
+def tigre_row_weights(views, nu, nv, vol: Volume, forward="joseph") -> np.ndarray:
+    """W = 1 / A_lo 1 over the support grid, A_lo 1 <= min voxel / 2 -> 1e30
+    first (AlgebraicTigreBackend::iterate, YkAlgebraicBackends.hpp:178-186)."""
+    fwd, _ = _operators((forward, "joseph"))
+    r = fwd(_d(views), nu, nv, tigre_support(views, nv, vol), np.ones((2, 2, 2), dtype=np.float32))
+    return 1.0 / np.where(r <= min(vol.vx, vol.vy, vol.vz) / 2, 1e30, r)
+
+
+def algebraic(views, nu, nv, vol: Volume, b, method="sart", weights=None, order=None, iterations=10,
+              subsets=1, relaxation=1.0, relaxation_reduction=1.0, epsilon=1e-6, min_value=None,
+              max_value=None, forward="joseph", back="joseph", tv_strength=0.0, tv_iterations=0,
+              tv_epsilon=1e-8, tv_strength_reduction=1.0, x0=None):
+    """FP64 restatement of the unified algebraic reconstructor over the
+    oracle's projector pair; returns the volume [z][y][x].
+
+    Outer loop (AlgebraicReconstructorEx::reconstruct,
+    src/Reconstruction/Iterative/Flat/YkAlgebraicReconstructorEx.hpp:137-180):
+    every subset once, then ``tv_iterations`` steps x -= strength grad TV
+    (SmoothedTv::apply, src/Reconstruction/Iterative/Regularization/
+    RegularizerPlugins.cpp:70-79) with strength *= reduction afterwards, then
+    the bounds.  Subset count: SIRT 1, SART one per view, OS-SART ``subsets``
+    (resolvedSubsetCount_, :357-363).
+
+    ``weights="detailed"`` (AlgebraicDetailedWeightBackend::iterate,
+    YkAlgebraicBackends.hpp:706-790; one subset: SirtBatchBackend::iterate,
+    YkSirtBatchBackend.hpp:46-78): subset s = views s, s + n, ...; R = A_s 1,
+    r = R > eps ? (b_s - A_s x) / R : 0; C = B_s 1 (one subset: C <= 0 ->
+    1e30); x += lambda B_s r / (C + eps), bounds; lambda *= reduction^(1/n)
+    after each update (:117-121 of the reconstructor).
+
+    ``weights="tigre"`` (AlgebraicTigreBackend::init/iterate,
+    YkAlgebraicBackends.hpp:64-228): contiguous blocks of ceil(Na / n) views;
+    W = 1 / A_lo 1 where A_lo projects a 2 x 2 x 2 support volume of voxel
+    (1.1 Nx vx / 2, 1.1 Ny vy / 2, max(Nv dv, Nz vz) / 2) centred on the
+    volume, with A_lo 1 <= min voxel / 2 -> 1e30 before the reciprocal;
+    C = B_s 1 with C <= 0 -> 1e30; r = (b_s - A_s x) W; x += lambda B_s r /
+    (C + eps), bounds; lambda *= reduction once per sweep.
+    """
+    views = _d(views)
+    na = views.shape[0]
+    if method not in ALGEBRAIC_METHODS:
+        raise ValueError(method)
+    weights = weights or ("tigre" if method == "sart" else "detailed")
+    order = order or ("golden" if method == "ossart" else "sequential")
+    n = 1 if method == "sirt" else na if method == "sart" else int(subsets)
+    fwd, bwd = _operators((forward, back))
+    A = lambda vs, x: fwd(vs, nu, nv, vol, x.astype(np.float32))  # noqa: E731
+    B = lambda vs, p: bwd(vs, nu, nv, vol, p)  # noqa: E731
+    b = _d(b).reshape(na, nv, nu)
+    x = np.zeros((vol.nz, vol.ny, vol.nx)) if x0 is None else _d(x0).copy()
+    tigre = weights == "tigre"
+    if tigre:
+        bs = (na + n - 1) // n
+        units = [list(range(s, min(na, s + bs))) for s in range(0, na, bs)]
+        visit = list(range(len(units)))
+    else:
+        units = [list(range(s, na, n)) for s in range(n)]
+        visit = golden_subset_order(n) if order == "golden" and n > 1 else list(range(n))
+    mask_cols = tigre or len(units) == 1
+
+    rows, cols = [], []
+    for u in units:
+        vs = views[u]
+        if tigre:
+            r = tigre_row_weights(vs, nu, nv, vol, forward)
+        else:
+            r = A(vs, np.ones_like(x))
+        c = B(vs, np.ones((len(u), nv, nu)))
+        if mask_cols:
+            c = np.where(c <= 0.0, 1e30, c)
+        rows.append(r)
+        cols.append(c)
+
+    def bounds(v):
+        if min_value is not None:
+            v = np.maximum(v, min_value)
+        if max_value is not None:
+            v = np.minimum(v, max_value)
+        return v
+
+    lam = float(relaxation)
+    per_update = 1.0 if tigre else float(relaxation_reduction) ** (1.0 / len(units))
+    strength = float(tv_strength)
+    for _ in range(iterations):
+        for k in visit:
+            u = units[k]
+            res = b[u] - A(views[u], x)
+            if tigre:
+                res = res * rows[k]
+            else:
+                res = np.where(rows[k] > epsilon, res / np.where(rows[k] > epsilon, rows[k], 1.0), 0.0)
+            x = bounds(x + lam * B(views[u], res) / (cols[k] + epsilon))
+            lam *= per_update
+        if tigre:
+            lam *= relaxation_reduction
+        if tv_iterations > 0 and strength > 0:
+            for _ in range(tv_iterations):
+                _, g = tv_value_grad(x, vol, tv_epsilon)
+                x = x - strength * g
+            strength *= tv_strength_reduction
+        x = bounds(x)
+    return x
