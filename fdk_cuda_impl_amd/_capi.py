@@ -21,7 +21,7 @@ STATUS_NAMES = {0: "None", 1: "InvalidConfig", 2: "InvalidGeometry", 3: "Unsuppo
 HOST, DEVICE = 0, 1
 F32, F16, BF16 = 0, 1, 2
 FP_JOSEPH, FP_SIDDON = 1, 2
-BP_JOSEPH_ADJOINT, BP_SIDDON_ADJOINT, BP_FDK, BP_FDK_PSEUDO_MATCHED = 8, 4, 6, 7
+BP_JOSEPH_ADJOINT, BP_SIDDON_ADJOINT, BP_FDK, BP_FDK_PSEUDO_MATCHED, BP_JOSEPH_VOXEL = 8, 4, 6, 7, 10
 PARKER_AUTO, PARKER_DISABLED, PARKER_ENABLED = 0, 1, 2
 FILTERS = {"none": 0, "ramlak": 1, "shepplogan": 2, "cosine": 3, "hann": 4, "hamming": 5,
            "blackman": 6, "butterworth": 7, "kaiser": 8, "tukey": 9}

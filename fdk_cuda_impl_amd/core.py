@@ -402,7 +402,7 @@ def build_views_from_system(system: C.System) -> np.ndarray:
 # ---------------------------------------------------------------------------
 FORWARD = {"joseph": C.FP_JOSEPH, "siddon": C.FP_SIDDON}
 BACK = {"joseph": C.BP_JOSEPH_ADJOINT, "siddon": C.BP_SIDDON_ADJOINT,
-        "fdk": C.BP_FDK, "fdk_pseudo_matched": C.BP_FDK_PSEUDO_MATCHED}
+        "fdk": C.BP_FDK, "fdk_pseudo_matched": C.BP_FDK_PSEUDO_MATCHED, "joseph_voxel": C.BP_JOSEPH_VOXEL}
 ALGEBRAIC = {"sirt": 0, "sart": 1, "ossart": 2}  # yk_algebraic_method
 WEIGHT_MODELS = {"detailed": 0, "tigre": 1}     # yk_weight_model
 SUBSET_ORDERS = {"sequential": 0, "golden": 1}  # yk_subset_order
@@ -472,7 +472,7 @@ class Projector:
 
     def algebraic(self, b, method="sart", x=None, iterations=10, relaxation=1.0, subsets=1,
                   weights=None, order=None, relaxation_reduction=1.0, epsilon=1e-6, min_value=None,
-                  max_value=None, forward="joseph", back="joseph", tv_strength=0.0, tv_iterations=0,
+                  max_value=None, forward="joseph", back="joseph_voxel", tv_strength=0.0, tv_iterations=0,
                   tv_epsilon=1e-8, tv_strength_reduction=1.0):
         """SIRT / SART / OS-SART on the device, updating x (zeros when None) in place.
 
@@ -480,8 +480,10 @@ class Projector:
         "tigre" (TIGRE-style approximate row weights over contiguous view
         blocks); the defaults follow the reference's public entry points:
         SART uses "tigre", SIRT and OS-SART "detailed", OS-SART with the
-        golden-ratio subset order (YkAlgebraicReconstructorEx.hpp:15-180,
-        YkAlgebraicBackends.hpp:32-310, 597-800).
+        golden-ratio subset order, and the model pair defaults to Joseph with
+        the voxel-driven Joseph backprojection (FP_Joseph / BP_Joseph_v3)
+        (YkAlgebraicReconstructorEx.hpp:15-180, YkAlgebraicBackends.hpp:32-310,
+        597-800).
         """
         torch = _torch()
         if method not in ALGEBRAIC:

@@ -1001,8 +1001,13 @@ siddon_adjoint_sep_kernel(const float2* __restrict__ col_xy, const float2* __res
 // outside the detector), per-sample weight
 //   FDK:     SOD^2 dtheta fscale / (Cc . [X, 1])^2,  Cc = (-src / SOD, SOD)
 //   matched: L^3 / (|X - src|^2 SDD_plane) * vx vy vz / (du dv),
-//            L^2 = |detS + u detU + v detV - src|^2.
+//            L^2 = |detS + u detU + v detV - src|^2;
+//   voxel-driven Joseph (joseph_bp_v3_kernel,
+//   src/FlatFpBp/BP/kernels/YkBPJosephKernel.cu:624-735): the physical length
+//   of X - src through one voxel along its main axis m = argmax |d_k / v_k|
+//   (ties X, then Y): v_m |d| / |d_m|.
 constexpr int kOpBpZ = 8;
+enum OpBpMode { kOpFdk = 0, kOpMatched = 1, kOpJosephVoxel = 2 };
 
 __device__ __forceinline__ float bilinear_view(const float* __restrict__ p, int nu, int nv, float u, float v)
 {
@@ -1018,7 +1023,7 @@ __device__ __forceinline__ float bilinear_view(const float* __restrict__ p, int 
     return (1.f - b) * ((1.f - a) * t00 + a * t01) + b * ((1.f - a) * t10 + a * t11);
 }
 
-template <bool MATCHED>
+template <int MODE>
 __global__ void __launch_bounds__(256)
 fdk_operator_bp_kernel(const float* __restrict__ proj, float* __restrict__ volume,
     const OpBpViewDev* __restrict__ views, VolumeGeom g, int nu, int nv, int count, float scale,
@@ -1044,7 +1049,14 @@ fdk_operator_bp_kernel(const float* __restrict__ proj, float* __restrict__ volum
             const float u = (c.cu[0] * X + c.cu[1] * Y + c.cu[2] * Z + c.cu[3]) / den;
             const float v = (c.cv[0] * X + c.cv[1] * Y + c.cv[2] * Z + c.cv[3]) / den;
             float w;
-            if constexpr (MATCHED) {
+            if constexpr (MODE == kOpJosephVoxel) {
+                const float dx = X - c.src[0], dy = Y - c.src[1], dz = Z - c.src[2];
+                const float ax = fabsf(dx / g.vx), ay = fabsf(dy / g.vy), az = fabsf(dz / g.vz);
+                const bool mx = ax >= ay && ax >= az, my = !mx && ay >= az;
+                const float vm = mx ? g.vx : my ? g.vy : g.vz;
+                const float dm = mx ? dx : my ? dy : dz;
+                w = vm * sqrtf(dx * dx + dy * dy + dz * dz) / fabsf(dm);
+            } else if constexpr (MODE == kOpMatched) {
                 const float L2 = c.l2[0] + u * (c.l2[1] + u * c.l2[3]) + v * (c.l2[2] + u * c.l2[4] + v * c.l2[5]);
                 if (!(L2 >= 1e-8f)) continue;
                 const float dx = X - c.src[0], dy = Y - c.src[1], dz = Z - c.src[2];
@@ -1162,11 +1174,14 @@ void launch_siddon_adjoint(const SiddonAdjointLaunch& p, hipStream_t s)
 void launch_fdk_operator_bp(const OpBpLaunch& p, hipStream_t s)
 {
     dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, (p.vol.nz + kOpBpZ - 1) / kOpBpZ);
-    if (p.matched)
-        hipLaunchKernelGGL(fdk_operator_bp_kernel<true>, grid, block, 0, s, p.proj, p.volume, p.views, p.vol,
-            p.nu, p.nv, p.count, p.scale, p.accumulate);
+    if (p.mode == kOpMatched)
+        hipLaunchKernelGGL(fdk_operator_bp_kernel<kOpMatched>, grid, block, 0, s, p.proj, p.volume, p.views,
+            p.vol, p.nu, p.nv, p.count, p.scale, p.accumulate);
+    else if (p.mode == kOpJosephVoxel)
+        hipLaunchKernelGGL(fdk_operator_bp_kernel<kOpJosephVoxel>, grid, block, 0, s, p.proj, p.volume, p.views,
+            p.vol, p.nu, p.nv, p.count, p.scale, p.accumulate);
     else
-        hipLaunchKernelGGL(fdk_operator_bp_kernel<false>, grid, block, 0, s, p.proj, p.volume, p.views, p.vol,
+        hipLaunchKernelGGL(fdk_operator_bp_kernel<kOpFdk>, grid, block, 0, s, p.proj, p.volume, p.views, p.vol,
             p.nu, p.nv, p.count, p.scale, p.accumulate);
     YK_HIP(hipGetLastError());
 }

@@ -264,7 +264,7 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
         p.nu = nu; p.nv = nv; p.count = count;
         p.accumulate = accumulate;
         launch_siddon_adjoint(p, s);
-    } else if (kind == YK_BP_FDK || kind == YK_BP_FDK_PSEUDO_MATCHED) {
+    } else if (kind == YK_BP_FDK || kind == YK_BP_FDK_PSEUDO_MATCHED || kind == YK_BP_JOSEPH_VOXEL) {
         OpBpLaunch p{};
         p.proj = d_proj;
         p.volume = d_volume;
@@ -272,7 +272,7 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
         p.vol = vol;
         p.nu = nu; p.nv = nv; p.count = count;
         p.scale = float(op_scale[first]);
-        p.matched = kind == YK_BP_FDK_PSEUDO_MATCHED ? 1 : 0;
+        p.mode = kind == YK_BP_FDK_PSEUDO_MATCHED ? 1 : kind == YK_BP_JOSEPH_VOXEL ? 2 : 0;
         p.accumulate = accumulate;
         launch_fdk_operator_bp(p, s);
     } else {

@@ -370,7 +370,7 @@ struct OpBpLaunch {
     VolumeGeom vol;
     int nu, nv, count;
     float scale;                // vx vy vz / (du dv) of the first view (matched weight)
-    int matched;
+    int mode;                   // 0 FDK weight, 1 pseudo-matched, 2 voxel-driven Joseph
     int accumulate;
 };
 void launch_fdk_operator_bp(const OpBpLaunch& p, hipStream_t s);

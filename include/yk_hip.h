@@ -80,10 +80,15 @@ typedef enum yk_forward_kind { YK_FP_JOSEPH = 1, YK_FP_SIDDON = 2 } yk_forward_k
  * src/FlatFpBp/BP/kernels/YkBpFdkKernel.cu:228-557): weight
  * SOD^2 dtheta fscale / (SOD - src.X / SOD)^2, or the pseudo-matched
  * L^3 / (|X - src|^2 SDD_plane) * vx vy vz / (du dv).  Derived geometry
- * (dtheta) is taken over the projector's whole view sequence. */
+ * (dtheta) is taken over the projector's whole view sequence.
+ * YK_BP_JOSEPH_VOXEL is the voxel-driven Joseph backprojection the reference's
+ * algebraic solvers use as B by default (BP_Joseph_v3 / v2,
+ * src/FlatFpBp/BP/kernels/YkBPJosephKernel.cu:430-735): the bilinear sample at
+ * the projected voxel centre times the ray's length through one voxel along
+ * its main axis; not the exact transpose of YK_FP_JOSEPH. */
 typedef enum yk_back_kind {
     YK_BP_JOSEPH_ADJOINT = 8, YK_BP_SIDDON_ADJOINT = 4,
-    YK_BP_FDK = 6, YK_BP_FDK_PSEUDO_MATCHED = 7
+    YK_BP_FDK = 6, YK_BP_FDK_PSEUDO_MATCHED = 7, YK_BP_JOSEPH_VOXEL = 10
 } yk_back_kind;
 
 /* ------------------------------------------------------------------------ */

@@ -726,7 +726,12 @@ void yko_bp_fdk(const double* coeffs, int n_views, const double* filtered,
  *   matched: w = L^3 / (|X - src|^2 |SDD_plane|) * vx vy vz / (du0 dv0),
  *            L = |detS + u detU + v detV - src|, du0/dv0 of the first view
  * A sample is skipped when den < 1e-8, |SOD - src.X/SOD| < 1e-8, L^2 < 1e-8
- * or |X - src|^2 < 1e-8.  out ([nz][ny][nx]) is overwritten.
+ * or |X - src|^2 < 1e-8.  matched = 2 selects the voxel-driven Joseph model
+ * (joseph_bp_v3_kernel, src/FlatFpBp/BP/kernels/YkBPJosephKernel.cu:624-735;
+ * the same model as joseph_bp_v2_kernel, :430-560): w = the physical length
+ * of the ray X - src through one voxel along its main axis, m = argmax
+ * |d_k / vox_k| (ties X, then Y), w = vox_m |d| / |d_m|, skipped only when
+ * den < 1e-8.  out ([nz][ny][nx]) is overwritten.
  */
 void yko_bp_fdk_operator(const double* views, const double* derived, const double* coeffs,
     int n_views, const double* proj, int nu, int nv, const yko_volume* vol, int matched,
@@ -755,7 +760,14 @@ void yko_bp_fdk_operator(const double* views, const double* derived, const doubl
             const double u = (c[YKO_C_CU + 3] + X[0] * c[YKO_C_CU] + X[1] * c[YKO_C_CU + 1] + X[2] * c[YKO_C_CU + 2]) / den;
             const double v = (c[YKO_C_CV + 3] + X[0] * c[YKO_C_CV] + X[1] * c[YKO_C_CV + 1] + X[2] * c[YKO_C_CV + 2]) / den;
             double w;
-            if (matched) {
+            if (matched == 2) {
+                const double d3[3] = { X[0] - g[0], X[1] - g[1], X[2] - g[2] };
+                const double vox3[3] = { vol->vx, vol->vy, vol->vz };
+                const double ax = fabs(d3[0] / vox3[0]), ay = fabs(d3[1] / vox3[1]), az = fabs(d3[2] / vox3[2]);
+                const int m = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+                const double len = sqrt(d3[0] * d3[0] + d3[1] * d3[1] + d3[2] * d3[2]);
+                w = vox3[m] * len / fabs(d3[m]);
+            } else if (matched) {
                 double P[3], L2 = 0.0, lsq = 0.0;
                 for (int a = 0; a < 3; ++a) {
                     P[a] = g[3 + a] + u * g[6 + a] + v * g[9 + a] - g[a];

@@ -106,8 +106,9 @@ double yko_bilinear(const double* plane, int nu, int nv, double u, double v);
 void yko_bp_fdk(const double* coeffs, int n_views, const double* filtered,
     int nu, int nv, const yko_volume* vol, const int64_t* voxel_index,
     int64_t n_voxels, double* out);
-/* operator-form FDK backprojections (weight FDK or pseudo-matched) of
- * unfiltered views (src/FlatFpBp/BP/kernels/YkBpFdkKernel.cu:228-466) */
+/* operator-form voxel-driven backprojections of unfiltered views: matched 0
+ * FDK weight, 1 pseudo-matched (src/FlatFpBp/BP/kernels/YkBpFdkKernel.cu:228-466),
+ * 2 voxel-driven Joseph (YkBPJosephKernel.cu:624-735) */
 void yko_bp_fdk_operator(const double* views, const double* derived, const double* coeffs,
     int n_views, const double* proj, int nu, int nv, const yko_volume* vol, int matched,
     double* out);

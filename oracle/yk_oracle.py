@@ -239,8 +239,9 @@ def bp_fdk(coeffs_, filtered, vol: Volume, voxel_index=None) -> np.ndarray:
 
 
 def bp_fdk_operator(views, proj, vol: Volume, matched=False, scan_range=None) -> np.ndarray:
-    """Operator-form FDK backprojection of unfiltered views (weight FDK or
-    pseudo-matched), derived geometry over the whole view sequence."""
+    """Operator-form voxel-driven backprojection of unfiltered views: weight
+    FDK (matched False/0), pseudo-matched (True/1) or the voxel-driven Joseph
+    chord (2); derived geometry over the whole view sequence."""
     views = _d(views)
     p = _d(proj)
     n, nv, nu = p.shape
@@ -248,7 +249,7 @@ def bp_fdk_operator(views, proj, vol: Volume, matched=False, scan_range=None) ->
     c = coeffs(views, d)
     out = np.zeros((vol.nz, vol.ny, vol.nx))
     lib().yko_bp_fdk_operator(_dp(views), _dp(d), _dp(c), n, _dp(p), nu, nv, ctypes.byref(vol),
-                              1 if matched else 0, _dp(out))
+                              int(matched), _dp(out))
     return out
 
 
@@ -357,9 +358,18 @@ def golden_subset_order(n):
     return [(i * step) % n for i in range(n)]
 
 
+def _voxel_driven(mode):
+    def back(views, nu, nv, vol, proj):
+        return bp_fdk_operator(views, _d(proj).reshape(-1, nv, nu), vol, matched=mode)
+    return back
+
+
 def _operators(kind):
+    """Forward and back models by name; the voxel-driven backs are the
+    pseudo-matched FDK weight and the voxel-driven Joseph chord."""
     fwd = {"joseph": fp_joseph, "siddon": fp_siddon}[kind[0]]
-    back = {"joseph": bp_joseph_adjoint, "siddon": bp_siddon_adjoint}[kind[1]]
+    back = {"joseph": bp_joseph_adjoint, "siddon": bp_siddon_adjoint,
+            "fdk_pseudo_matched": _voxel_driven(1), "joseph_voxel": _voxel_driven(2)}[kind[1]]
     return fwd, back
 
 
@@ -382,7 +392,7 @@ def tigre_row_weights(views, nu, nv, vol: Volume, forward="joseph") -> np.ndarra
 
 def algebraic(views, nu, nv, vol: Volume, b, method="sart", weights=None, order=None, iterations=10,
               subsets=1, relaxation=1.0, relaxation_reduction=1.0, epsilon=1e-6, min_value=None,
-              max_value=None, forward="joseph", back="joseph", tv_strength=0.0, tv_iterations=0,
+              max_value=None, forward="joseph", back="joseph_voxel", tv_strength=0.0, tv_iterations=0,
               tv_epsilon=1e-8, tv_strength_reduction=1.0, x0=None):
     """FP64 restatement of the unified algebraic reconstructor over the
     oracle's projector pair; returns the volume [z][y][x].

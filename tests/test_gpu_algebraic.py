@@ -42,18 +42,21 @@ def run_both(oracle, problem, **kw):
     return got, ref
 
 
+@pytest.mark.parametrize("back", ["joseph_voxel", "joseph"])
 @pytest.mark.parametrize("method,weights", [
     ("sirt", "detailed"), ("sart", "detailed"), ("ossart", "detailed"), ("ossart", "tigre")])
-def test_reference_smoke_cases(oracle, problem, method, weights):
+def test_reference_smoke_cases(oracle, problem, method, weights, back):
+    """FP_Joseph with BP_Joseph_v3 (the fixture's pair), and with the exact
+    Joseph transpose."""
     got, ref = run_both(oracle, problem, method=method, weights=weights, iterations=1, subsets=2,
-                        relaxation=0.2, min_value=0.0)
+                        relaxation=0.2, min_value=0.0, back=back)
     assert np.isfinite(got).all() and np.abs(got).max() > 1e-8
     assert rel_rmse(got, ref) < TOL, rel_rmse(got, ref)
 
 
 def test_reference_smoothed_tv_case(oracle, problem):
     got, ref = run_both(oracle, problem, method="ossart", weights="detailed", iterations=2, subsets=2,
-                        relaxation=0.2, min_value=0.0, tv_strength=1e-3, tv_iterations=2,
+                        relaxation=0.2, min_value=0.0, back="joseph_voxel", tv_strength=1e-3, tv_iterations=2,
                         tv_epsilon=1e-4, tv_strength_reduction=0.95)
     assert rel_rmse(got, ref) < TOL, rel_rmse(got, ref)
 
@@ -64,7 +67,14 @@ def test_reference_smoothed_tv_case(oracle, problem):
     dict(method="ossart", subsets=4, order="golden", relaxation_reduction=0.8),
     dict(method="ossart", subsets=3, weights="tigre", relaxation_reduction=0.8, max_value=0.5),
     dict(method="ossart", subsets=4, forward="siddon", back="siddon", min_value=0.0),
-], ids=["sart-tigre", "sirt-tigre", "ossart-golden", "ossart3-tigre", "ossart-siddon"])
+    # the reference's defaults: FP_Joseph with the voxel-driven BP_Joseph_v3
+    dict(method="sirt", back="joseph_voxel", min_value=0.0),
+    dict(method="ossart", subsets=2, back="joseph_voxel", relaxation=0.2),
+    # the Session's Automatic pair for algebraic pipelines: Siddon + FDK pseudo-matched
+    dict(method="sart", forward="siddon", back="fdk_pseudo_matched", relaxation=0.5),
+    dict(method="ossart", subsets=4, forward="siddon", back="fdk_pseudo_matched", min_value=0.0),
+], ids=["sart-tigre", "sirt-tigre", "ossart-golden", "ossart3-tigre", "ossart-siddon", "sirt-josephv3",
+        "ossart-josephv3", "sart-automatic", "ossart-automatic"])
 def test_schedules(oracle, problem, kw):
     got, ref = run_both(oracle, problem, iterations=3, **kw)
     assert rel_rmse(got, ref) < TOL, rel_rmse(got, ref)
@@ -74,7 +84,7 @@ def test_sart_entry_point_is_detailed_per_view(oracle, problem):
     g, P, b, ov, vol = problem
     got = P.sart(b, iterations=2, relaxation=0.5, nonnegative=True).cpu().numpy()
     ref = oracle.algebraic(ov, g.nu, g.nv, vol, b.cpu().numpy(), method="sart", weights="detailed",
-                           iterations=2, relaxation=0.5, min_value=0.0)
+                           iterations=2, relaxation=0.5, min_value=0.0, back="joseph")
     assert rel_rmse(got, ref) < TOL, rel_rmse(got, ref)
 
 

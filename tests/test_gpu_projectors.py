@@ -74,11 +74,12 @@ def test_adjoint_matches_oracle_transpose(ctx, oracle, kind, name):
     assert rel_rmse(got, ref) < 2e-5, rel_rmse(got, ref)
 
 
-@pytest.mark.parametrize("kind", ["fdk", "fdk_pseudo_matched"])
+@pytest.mark.parametrize("kind", ["fdk", "fdk_pseudo_matched", "joseph_voxel"])
 @pytest.mark.parametrize("name", ["plain", "aniso"])
 def test_operator_fdk_backprojection_matches_oracle(ctx, oracle, kind, name):
-    """Operator-form FDK backprojections (iterative B), source lifted out of the
-    orbit plane so the FDK weight uses its own denominator."""
+    """Voxel-driven backprojections used as the iterative B (FDK weight,
+    pseudo-matched, voxel-driven Joseph), source lifted out of the orbit plane
+    so the FDK weight uses its own denominator."""
     g = small_geometry(source_offset_mm=(0.0, 0.0, 2.0), **GEOMS[name])
     P = yk.Projector(ctx, g.views(), g.nu, g.nv, g.grid())
     rng = np.random.default_rng(14)
@@ -86,7 +87,7 @@ def test_operator_fdk_backprojection_matches_oracle(ctx, oracle, kind, name):
     got = P.adjoint(torch.from_numpy(y).cuda(), kind=kind).cpu().numpy()
     ov = oracle.views_from_float(P.views)
     ref = oracle.bp_fdk_operator(ov, y.astype(np.float64), oracle_volume(oracle, g),
-                                 matched=kind == "fdk_pseudo_matched")
+                                 matched={"fdk": 0, "fdk_pseudo_matched": 1, "joseph_voxel": 2}[kind])
     assert rel_rmse(got, ref) < 1e-5, rel_rmse(got, ref)
     # accumulate adds onto the volume
     acc = P.adjoint(torch.from_numpy(y).cuda(), kind=kind, out=torch.ones(P.volume_shape, device="cuda"),

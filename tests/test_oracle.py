@@ -562,7 +562,7 @@ def test_tv_value_and_gradient(oracle):
         assert abs(fd - g[k]) < 1e-6 * max(1.0, abs(fd))
 
 
-@pytest.mark.parametrize("matched", [False, True])
+@pytest.mark.parametrize("matched", [False, True, 2])
 def test_bp_fdk_operator_brute_force(oracle, matched):
     """Operator-form FDK backprojections (src/FlatFpBp/BP/kernels/YkBpFdkKernel.cu:228-466)
     with a source above the orbit plane, so the FDK weight's denominator
@@ -583,7 +583,13 @@ def test_bp_fdk_operator_brute_force(oracle, matched):
                 for i, g in enumerate(v):
                     src = g[0:3]
                     u, vv = project_point(g, X)
-                    if matched:
+                    if matched == 2:
+                        # ray-plane intersection form (joseph_bp_v2_kernel): length of the
+                        # line through one slab of the main axis in voxel units
+                        dvox = (X - src) / np.array([1.5, 1.4, 1.3])
+                        m = int(np.argmax(np.abs(dvox)))
+                        w = np.linalg.norm(X - src) * (np.array([1.5, 1.4, 1.3])[m] / abs((X - src)[m]))
+                    elif matched:
                         L = np.linalg.norm(g[3:6] + u * g[6:9] + vv * g[9:12] - src)
                         m = np.cross(g[6:9], g[9:12])
                         sdd = abs(np.dot(g[3:6] - src, m / np.linalg.norm(m)))
@@ -594,6 +600,23 @@ def test_bp_fdk_operator_brute_force(oracle, matched):
                     acc += interp2(p[i], u, vv)[0] * w
                 want[z, y, x] = acc
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12)
+
+
+def test_voxel_driven_joseph_weight_is_the_joseph_matrix_entry(oracle):
+    """For a ray through a voxel centre the Joseph model's entry A[ray, voxel]
+    is its step length (interpolation weight 1), and the voxel-driven Joseph
+    backprojection of that pixel's delta samples weight 1 at the voxel: both
+    equal, with the main axis X or Y and anisotropic voxels."""
+    n, nu, nv = 4, 21, 15
+    v = oracle.build_views(n, n, 60.0, 110.0, nu, nv, 1.0, 1.0, start_angle=0.3)
+    vol = oracle.volume(5, 5, 3, 1.0, 1.2, 0.9)
+    for i in range(n):
+        e = np.zeros((n, nv, nu))
+        e[i, (nv - 1) // 2, (nu - 1) // 2] = 1.0
+        a = oracle.bp_joseph_adjoint(v, nu, nv, vol, e)[1, 2, 2]
+        b = oracle.bp_fdk_operator(v, e, vol, matched=2)[1, 2, 2]
+        assert a > 0.9
+        assert b == pytest.approx(a, rel=1e-12)
 
 
 def test_bp_fdk_operator_equals_fdk_bp_in_the_orbit_plane(oracle):

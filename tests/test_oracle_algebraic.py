@@ -77,7 +77,8 @@ def test_sirt_reaches_the_weighted_least_squares_solution(oracle, tiny):
     R = A.sum(axis=1)
     w = np.where(R > 1e-6, 1.0 / np.where(R > 1e-6, R, 1.0), 0.0)
     xs = np.linalg.lstsq(np.sqrt(w)[:, None] * A, np.sqrt(w) * b, rcond=None)[0]
-    x = oracle.algebraic(views, NU, NV, vol, b, method="sirt", iterations=3000, relaxation=1.9)
+    x = oracle.algebraic(views, NU, NV, vol, b, method="sirt", iterations=3000, relaxation=1.9,
+                         back="joseph")
     assert np.linalg.norm(x.ravel() - xs) / np.linalg.norm(xs) < 1e-5
     # the unweighted and the R-multiplied solutions are far from it
     for alt in (np.ones_like(R), R):
@@ -94,7 +95,7 @@ def test_constant_volume_closed_form(oracle, tiny, method, subsets, order):
     b = oracle.fp_joseph(views, NU, NV, vol, np.full(SHAPE, c, np.float32))
     x = oracle.algebraic(views, NU, NV, vol, b, method=method, weights="detailed", order=order,
                          subsets=subsets, iterations=iters, relaxation=lam,
-                         relaxation_reduction=red)
+                         relaxation_reduction=red, back="joseph")
     n = {"sirt": 1, "sart": NA, "ossart": subsets}[method]
     # c - x_{k+1} = (1 - lambda q^k)(c - x_k), q = red^(1/n), x_0 = 0
     gap = c
@@ -107,7 +108,7 @@ def test_constant_volume_upper_bound(oracle, tiny):
     views, vol = tiny
     b = oracle.fp_joseph(views, NU, NV, vol, np.full(SHAPE, 0.7, np.float32))
     x = oracle.algebraic(views, NU, NV, vol, b, method="ossart", subsets=2, iterations=2,
-                         max_value=0.3, min_value=0.0)
+                         max_value=0.3, min_value=0.0, back="joseph")
     np.testing.assert_allclose(x, 0.3, rtol=0, atol=0)
 
 
