@@ -96,6 +96,30 @@ class AlgebraicDesc(ctypes.Structure):
                 ("tv_epsilon", ctypes.c_float), ("tv_strength_reduction", ctypes.c_float)]
 
 
+class CglsDesc(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("strategy", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("epsilon", ctypes.c_float),
+                ("restart_on_divergence", ctypes.c_int), ("use_min", ctypes.c_int),
+                ("min_value", ctypes.c_float), ("use_max", ctypes.c_int), ("max_value", ctypes.c_float),
+                ("forward_kind", ctypes.c_int), ("back_kind", ctypes.c_int),
+                ("relative_residual_tolerance", ctypes.c_float),
+                ("relative_update_tolerance", ctypes.c_float),
+                ("relative_improvement_tolerance", ctypes.c_float),
+                ("minimum_iterations", ctypes.c_int), ("check_interval", ctypes.c_int),
+                ("patience", ctypes.c_int)]
+
+
+class CglsStats(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("completed_iterations", ctypes.c_int),
+                ("convergence_checks", ctypes.c_int), ("stopped_by_breakdown", ctypes.c_int),
+                ("stopped_by_divergence", ctypes.c_int), ("stopped_by_relative_residual", ctypes.c_int),
+                ("stopped_by_relative_update", ctypes.c_int), ("stopped_by_stagnation", ctypes.c_int),
+                ("projection_residual_l2", ctypes.c_double),
+                ("relative_projection_residual", ctypes.c_double),
+                ("relative_volume_update", ctypes.c_double),
+                ("relative_residual_improvement", ctypes.c_double)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _F = ctypes.c_float
@@ -140,6 +164,7 @@ SIGNATURES = {
     "yk_project": (_I, [_P, _I, _P, _P, _I, _I, _I]),
     "yk_backproject": (_I, [_P, _I, _P, _P, _I, _I, _I]),
     "yk_cgls": (_I, [_P, _I, _I, _P, _P, _I, _DP]),
+    "yk_cgls_ex": (_I, [_P, ctypes.POINTER(CglsDesc), _P, _P, ctypes.POINTER(CglsStats), _DP]),
     "yk_sart": (_I, [_P, _I, _I, _P, _P, _I, _F, _I]),
     "yk_algebraic": (_I, [_P, ctypes.POINTER(AlgebraicDesc), _P, _P]),
     "yk_tv_gradient": (_I, [_P, ctypes.POINTER(VolumeGrid), _P, _F, _P, _DP]),

@@ -406,6 +406,7 @@ BACK = {"joseph": C.BP_JOSEPH_ADJOINT, "siddon": C.BP_SIDDON_ADJOINT,
 ALGEBRAIC = {"sirt": 0, "sart": 1, "ossart": 2}  # yk_algebraic_method
 WEIGHT_MODELS = {"detailed": 0, "tigre": 1}     # yk_weight_model
 SUBSET_ORDERS = {"sequential": 0, "golden": 1}  # yk_subset_order
+CGLS_STRATEGIES = {"robust": 0, "astra": 1}     # yk_cgls_strategy
 
 
 class Projector:
@@ -459,6 +460,45 @@ class Projector:
         C.call("yk_cgls", self.handle, FORWARD[forward], BACK[back], _ptr(b), _ptr(x), iterations,
                res.ctypes.data_as(C._DP))
         return x, res
+
+    def cgls_ex(self, b, x=None, iterations=50, strategy="robust", epsilon=1e-8, restart=True,
+                min_value=None, max_value=None, forward="joseph", back="joseph_voxel",
+                relative_residual=0.0, relative_update=0.0, relative_improvement=0.0,
+                minimum_iterations=1, check_interval=1, patience=1):
+        """CGLS with the reference's strategies: "robust" (true residual each
+        step, rollback and one restart on divergence, convergence monitor) or
+        "astra" (plain recursion); defaults follow CglsReconstructionConfig
+        (YkCglsReconstructorEx.hpp:15-27).  Returns (x, residual norms, stats)."""
+        torch = _torch()
+        _check_cuda(b, torch.float32, (self.n_views, self.nv, self.nu), "projection")
+        if x is None:
+            x = torch.zeros(self.volume_shape, dtype=torch.float32, device=b.device)
+        _check_cuda(x, torch.float32, self.volume_shape, "volume")
+        d = C.CglsDesc()
+        d.struct_size = ctypes.sizeof(C.CglsDesc)
+        d.strategy = CGLS_STRATEGIES[strategy]
+        d.iterations = iterations
+        d.epsilon = epsilon
+        d.restart_on_divergence = 1 if restart else 0
+        d.use_min = 0 if min_value is None else 1
+        d.min_value = 0.0 if min_value is None else min_value
+        d.use_max = 0 if max_value is None else 1
+        d.max_value = 0.0 if max_value is None else max_value
+        d.forward_kind = FORWARD[forward]
+        d.back_kind = BACK[back]
+        d.relative_residual_tolerance = relative_residual
+        d.relative_update_tolerance = relative_update
+        d.relative_improvement_tolerance = relative_improvement
+        d.minimum_iterations = minimum_iterations
+        d.check_interval = check_interval
+        d.patience = patience
+        st = C.CglsStats()
+        st.struct_size = ctypes.sizeof(C.CglsStats)
+        res = np.zeros(max(iterations, 0) + 1, dtype=np.float64)
+        C.call("yk_cgls_ex", self.handle, ctypes.byref(d), _ptr(b), _ptr(x), ctypes.byref(st),
+               res.ctypes.data_as(C._DP))
+        stats = {name: getattr(st, name) for name, _ in C.CglsStats._fields_ if name != "struct_size"}
+        return x, res, stats
 
     def sart(self, b, x=None, iterations=1, relaxation=1.0, nonnegative=False, forward="joseph",
              back="joseph"):

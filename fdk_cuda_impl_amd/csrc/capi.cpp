@@ -449,6 +449,19 @@ yk_status yk_cgls(yk_projector* p, int fk, int bk, const float* d_b, float* d_x,
     });
 }
 
+yk_status yk_cgls_ex(yk_projector* p, const yk_cgls_desc* desc, const float* d_b, float* d_x, yk_cgls_stats* stats,
+    double* residual_norms)
+{
+    return guard([&] {
+        require(p != nullptr && desc != nullptr, YK_ERR_INVALID_CONFIG, "null projector or descriptor");
+        require(desc->struct_size >= sizeof(yk_cgls_desc), YK_ERR_INVALID_CONFIG,
+            "yk_cgls_desc is truncated or from an older layout");
+        require(d_b && d_x, YK_ERR_INVALID_BUFFER, "null device buffer");
+        run_cgls(p, *desc, d_b, d_x, stats, residual_norms);
+        p->ctx->finish();
+    });
+}
+
 // SART: view-by-view updates with exact per-view weights, the DETAILED
 // SART schedule of yk_algebraic (AlgebraicReconstructorEx per-view order,
 // src/Reconstruction/Iterative/Flat/YkAlgebraicReconstructorEx.hpp:137-180)

@@ -291,6 +291,56 @@ yk_status yk_backproject(yk_projector* projector, int back_kind, const float* d_
  * residual_norms (optional, iterations+1 doubles) receives ||b - A x_k||. */
 yk_status yk_cgls(yk_projector* projector, int forward_kind, int back_kind,
     const float* d_b, float* d_x, int iterations, double* residual_norms);
+/* CGLS with the reference's strategies (CglsReconstructorEx,
+ * src/Reconstruction/Iterative/Flat/YkCglsReconstructorEx.hpp:10-100):
+ *   ROBUST_RESTART (CglsRobustBackend, YkCglsBackends.hpp:222-560): after each
+ *     step the true residual ||b - A x||; a step that raises it is rolled back
+ *     and the recursion restarts from there once (a second divergence in a row,
+ *     or any with restart_on_divergence = 0, stops); optional convergence
+ *     monitor (IterativeConvergenceConfig, YkIterativeConvergence.hpp:9-47);
+ *   ASTRA (CglsAstraBackend, :22-190): the plain recursion.
+ * Both stop when ||A p||^2 < epsilon, use beta = gamma1 / (gamma + epsilon) and
+ * clamp x to the bounds after each iteration.  x is the start and the result.
+ * residual_norms (optional, iterations + 1 doubles): [0] = ||b - A x0||, then
+ * per completed iteration the true (ROBUST) or recursive (ASTRA) residual,
+ * repeated past an early stop. */
+typedef enum yk_cgls_strategy { YK_CGLS_ROBUST_RESTART = 0, YK_CGLS_ASTRA = 1 } yk_cgls_strategy;
+typedef struct yk_cgls_desc {
+    uint32_t struct_size;            /* sizeof(yk_cgls_desc) */
+    int strategy;                    /* yk_cgls_strategy */
+    int iterations;                  /* >= 1 */
+    float epsilon;                   /* > 0, 1e-8 in the reference */
+    int restart_on_divergence;       /* ROBUST_RESTART only */
+    int use_min;
+    float min_value;
+    int use_max;
+    float max_value;
+    int forward_kind;                /* yk_forward_kind */
+    int back_kind;                   /* yk_back_kind */
+    /* convergence monitor (ROBUST_RESTART; all tolerances 0: off) */
+    float relative_residual_tolerance;     /* ||b - A x|| / ||b|| */
+    float relative_update_tolerance;       /* ||x - x_prev|| / ||x_prev|| */
+    float relative_improvement_tolerance;  /* residual improvement between checks */
+    int minimum_iterations;          /* >= 0 */
+    int check_interval;              /* >= 1 */
+    int patience;                    /* >= 1 checks in a row */
+} yk_cgls_desc;
+typedef struct yk_cgls_stats {
+    uint32_t struct_size;            /* caller sets sizeof(yk_cgls_stats) */
+    int completed_iterations;
+    int convergence_checks;
+    int stopped_by_breakdown;
+    int stopped_by_divergence;
+    int stopped_by_relative_residual;
+    int stopped_by_relative_update;
+    int stopped_by_stagnation;
+    double projection_residual_l2;          /* at the last check (NaN: none) */
+    double relative_projection_residual;
+    double relative_volume_update;
+    double relative_residual_improvement;
+} yk_cgls_stats;
+yk_status yk_cgls_ex(yk_projector* projector, const yk_cgls_desc* desc, const float* d_b, float* d_x,
+    yk_cgls_stats* stats, double* residual_norms);
 /* SART with view-by-view updates (AlgebraicReconstructorEx,
  * src/Reconstruction/Iterative/Flat/YkAlgebraicReconstructorEx.hpp:137-180) */
 yk_status yk_sart(yk_projector* projector, int forward_kind, int back_kind,

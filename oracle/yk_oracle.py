@@ -484,3 +484,139 @@ def algebraic(views, nu, nv, vol: Volume, b, method="sart", weights=None, order=
             strength *= tv_strength_reduction
         x = bounds(x)
     return x
+
+
+# --- CGLS (robust-restart and ASTRA strategies) ------------------------------
+
+def cgls(views, nu, nv, vol: Volume, b, strategy="robust", iterations=50, epsilon=1e-8, restart=True,
+         min_value=None, max_value=None, forward="joseph", back="joseph_voxel", relative_residual=0.0,
+         relative_update=0.0, relative_improvement=0.0, minimum_iterations=1, check_interval=1,
+         patience=1, x0=None):
+    """FP64 restatement of CglsReconstructorEx over the oracle's projector
+    pair; returns (x, residual norms [iterations + 1], stats dict).
+
+    "astra" (CglsAstraBackend::run, src/Reconstruction/Iterative/Flat/
+    YkCglsBackends.hpp:84-156): r = b - A x, p = B r, gamma = |p|^2; per
+    iteration w = A p, stop if |w|^2 < eps, alpha = gamma / |w|^2, x += alpha p,
+    r -= alpha w, z = B r, beta = |z|^2 / (gamma + eps), p = z + beta p, bounds.
+    "robust" (CglsRobustBackend::run, :281-400, checkConvergence_ :454-500):
+    the same step, then the true residual |b - A x|; if it rose (after the first
+    iteration) x returns to its value before the step and, unless the previous
+    iteration restarted too or restarts are off, the recursion restarts from x;
+    then the convergence monitor on relative residual / update / improvement
+    every ``check_interval`` completed iterations, ``patience`` checks in a row.
+    """
+    views = _d(views)
+    fwd, bwd = _operators((forward, back))
+    A = lambda x: fwd(views, nu, nv, vol, x.astype(np.float32))  # noqa: E731
+    B = lambda p: bwd(views, nu, nv, vol, p)  # noqa: E731
+    b = _d(b).reshape(views.shape[0], nv, nu)
+    x = np.zeros((vol.nz, vol.ny, vol.nx)) if x0 is None else _d(x0).copy()
+    eps = float(epsilon)
+    monitor = relative_residual > 0 or relative_update > 0 or relative_improvement > 0
+    st = dict(completed_iterations=0, convergence_checks=0, stopped_by_breakdown=0, stopped_by_divergence=0,
+              stopped_by_relative_residual=0, stopped_by_relative_update=0, stopped_by_stagnation=0,
+              projection_residual_l2=np.nan, relative_projection_residual=np.nan,
+              relative_volume_update=np.nan, relative_residual_improvement=np.nan)
+
+    def bounds(v):
+        if min_value is not None:
+            v = np.maximum(v, min_value)
+        if max_value is not None:
+            v = np.minimum(v, max_value)
+        return v
+
+    def init(x):
+        r = b - A(x)
+        p = B(r)
+        return r, p, float(np.vdot(p, p))
+
+    res = np.zeros(iterations + 1)
+    r, p, gamma = init(x)
+    res[0] = np.linalg.norm(r)
+    if strategy == "astra":
+        k = 0
+        while k < iterations:
+            w = A(p)
+            ww = float(np.vdot(w, w))
+            if ww < eps:
+                st["stopped_by_breakdown"] = 1
+                break
+            alpha = gamma / ww
+            x = x + alpha * p
+            r = r - alpha * w
+            z = B(r)
+            g1 = float(np.vdot(z, z))
+            beta = g1 / (gamma + eps)
+            gamma = g1
+            p = z + beta * p
+            x = bounds(x)
+            st["completed_iterations"] = k + 1
+            res[k + 1] = np.linalg.norm(r)
+            k += 1
+        res[k + 1:] = res[k]
+        return x, res, st
+
+    measured = np.sqrt(max(0.0, float(np.vdot(b, b))))
+    re_init_at, count, l2_prev, prev_checked = -1, 0, np.inf, np.nan
+    k = 0
+    while k < iterations:
+        x_prev = x.copy()
+        q = A(p)
+        qq = float(np.vdot(q, q))
+        if qq < eps:
+            st["stopped_by_breakdown"] = 1
+            res[k + 1:] = res[k]
+            return x, res, st
+        alpha = gamma / qq
+        x = x + alpha * p
+        l2 = float(np.linalg.norm(b - A(x)))
+        if k > 0 and l2 > l2_prev:
+            x = x_prev
+            res[k + 1] = res[k]
+            if re_init_at + 1 == k or not restart:
+                st["stopped_by_divergence"] = 1
+                break
+            re_init_at = k
+            r, p, gamma = init(x)
+            l2_prev = np.inf
+            k += 1
+            continue
+        l2_prev = l2
+        r = r - alpha * q
+        s = B(r)
+        g1 = float(np.vdot(s, s))
+        beta = g1 / (gamma + eps)
+        gamma = g1
+        p = s + beta * p
+        x = bounds(x)
+        st["completed_iterations"] = k + 1
+        res[k + 1] = l2
+        done = k + 1
+        k += 1
+        if not monitor or (done % check_interval != 0 and done != iterations):
+            continue
+        st["convergence_checks"] += 1
+        st["projection_residual_l2"] = l2
+        st["relative_projection_residual"] = l2 / max(measured, eps)
+        residual_ok = relative_residual > 0 and st["relative_projection_residual"] <= relative_residual
+        stagnation_ok = False
+        if np.isfinite(prev_checked):
+            st["relative_residual_improvement"] = (prev_checked - l2) / max(prev_checked, eps)
+            stagnation_ok = (relative_improvement > 0 and st["relative_residual_improvement"] >= 0
+                             and st["relative_residual_improvement"] <= relative_improvement)
+        prev_checked = l2
+        update_ok = False
+        if relative_update > 0:
+            st["relative_volume_update"] = np.linalg.norm(x - x_prev) / max(np.linalg.norm(x_prev), eps)
+            update_ok = st["relative_volume_update"] <= relative_update
+        satisfied = done >= minimum_iterations and (residual_ok or update_ok or stagnation_ok)
+        count = count + 1 if satisfied else 0
+        if count >= patience:
+            st["stopped_by_relative_residual"] = int(residual_ok)
+            st["stopped_by_relative_update"] = int(update_ok)
+            st["stopped_by_stagnation"] = int(stagnation_ok)
+            break
+    last = min(k + 1, iterations) if st["stopped_by_divergence"] else min(k, iterations)
+    res[last + 1:] = res[last]
+    return x, res, st
