@@ -1,7 +1,7 @@
 // C++ Session facade (include/yk_session.hpp) over this library's C ABI: the
 // reference's ReconstructionSession (src/interface/YkReconstructionApi.cpp:
-// 133-418) with its FDK and ForwardProjection backends
-// (src/interface/YkSession.cpp:693-728, 881-935,
+// 133-418) with its FDK, ForwardProjection and flat algebraic / CGLS backends
+// (src/interface/YkSession.cpp:290-350, 693-728, 881-1030,
 // src/common/YkProjectionOperators.hpp:138-180) for flat detectors on
 // circular scans.  Validation order and error codes follow the reference:
 // struct_size / api_version, enum ranges, projector selection, pose, then
@@ -113,9 +113,11 @@ public:
         if (g.detector != EDetectorKind::Flat || g.trajectory != ETrajectoryKind::Circular)
             return fail(EApiErrorCode::UnsupportedCombination,
                         "this library provides the flat-detector circular-scan path only");
-        if (rec.pipeline != EPipeline::FDK && rec.pipeline != EPipeline::ForwardProjection)
+        const bool iterative = rec.pipeline == EPipeline::SART || rec.pipeline == EPipeline::SIRT ||
+                               rec.pipeline == EPipeline::OSSART || rec.pipeline == EPipeline::CGLS;
+        if (rec.pipeline != EPipeline::FDK && rec.pipeline != EPipeline::ForwardProjection && !iterative)
             return fail(EApiErrorCode::UnsupportedCombination,
-                        "the Session facade runs the FDK and ForwardProjection pipelines");
+                        "the Session facade runs the FDK, ForwardProjection, SART, SIRT, OSSART and CGLS pipelines");
         const SForwardProjectionPose& pose = g.forward_projection_pose;
         if (!std::isfinite(pose.rotation_x_rad) || !std::isfinite(pose.rotation_y_rad) ||
             !std::isfinite(pose.rotation_z_rad))
@@ -126,6 +128,37 @@ public:
             if (rec.forward_projector == EForwardProjector::SeparableFootprint)
                 return fail(EApiErrorCode::UnsupportedCombination, "the separable-footprint projector is not provided");
             forward_kind_ = rec.forward_projector == EForwardProjector::Joseph ? YK_FP_JOSEPH : YK_FP_SIDDON;
+        } else if (iterative) {
+            // resolveForwardProjector / resolveBackProjector
+            // (src/interface/YkProjectorSelection.hpp:11-44): Automatic is
+            // Siddon forward; backward the Siddon transpose where the solver
+            // needs A^T (CGLS), else the FDK pseudo-matched backprojection
+            if (rec.forward_projector == EForwardProjector::SeparableFootprint ||
+                rec.back_projector == EBackProjector::SeparableFootprint)
+                return fail(EApiErrorCode::UnsupportedCombination, "the separable-footprint projector is not provided");
+            forward_kind_ = rec.forward_projector == EForwardProjector::Joseph ? YK_FP_JOSEPH : YK_FP_SIDDON;
+            switch (rec.back_projector) {
+            case EBackProjector::Automatic:
+                back_kind_ = rec.pipeline == EPipeline::CGLS ? YK_BP_SIDDON_ADJOINT : YK_BP_FDK_PSEUDO_MATCHED;
+                break;
+            case EBackProjector::FdkPseudoMatched: back_kind_ = YK_BP_FDK_PSEUDO_MATCHED; break;
+            case EBackProjector::SiddonRayDriven: back_kind_ = YK_BP_SIDDON_ADJOINT; break;
+            case EBackProjector::Joseph: back_kind_ = YK_BP_JOSEPH_ADJOINT; break;
+            default: back_kind_ = YK_BP_JOSEPH_VOXEL; break;  // JosephV3
+            }
+            // the backends' own checks (AlgebraicReconstructorEx::validate_,
+            // CglsReconstructorEx::prepare); a failed backend initialisation
+            // is reported as InvalidGeometry
+            const SIterAlgoParams& it = rec.iterative;
+            subsets_ = it.subsets > 1 ? it.subsets : 1;
+            if (it.iterations <= 0 || (rec.pipeline != EPipeline::CGLS && !(it.relaxation > 0.f)) ||
+                !std::isfinite(it.relaxation) ||
+                (rec.pipeline == EPipeline::OSSART && subsets_ > g.circular.total_views))
+                return fail(EApiErrorCode::InvalidGeometry,
+                            "iterative backend initialisation failed: iterations, relaxation or subset count");
+            iterations_ = it.iterations;
+            relaxation_ = it.relaxation;
+            non_negative_ = rec.tigre.non_negative;
         } else {
             const int p = int(rec.fdk.parker.mode), f = int(rec.fdk.filter.type);
             if (p < 0 || p > 2 || f < 0 || f > 9)
@@ -198,7 +231,8 @@ public:
                 return r;
             }
         } else {
-            if (pose.enabled) rotate_views(views, pose, sys.volume.center_mm);
+            if (pose.enabled && rec.pipeline == EPipeline::ForwardProjection)
+                rotate_views(views, pose, sys.volume.center_mm);
             const float range = float(2.0 * 3.14159265358979323846 * c.total_views / c.views_per_turn);
             if (yk_status st = yk_projector_create(ctx_, views.data(), n, d.channels, d.rows, &sys.volume, range,
                                                    &proj_);
@@ -247,13 +281,22 @@ public:
         if (r.projection.element_count < view_elems_ * size_t(count) || r.volume.element_count < vol_elems_)
             return fail(EApiErrorCode::InvalidBuffer, "projection / volume element_count below what this call needs");
 
-        const bool ok = pipeline_ == EPipeline::FDK ? execute_fdk(r, offset, count) : execute_fp(r);
+        int done = views_, total = views_;
+        bool ok;
+        if (pipeline_ == EPipeline::FDK)
+            ok = execute_fdk(r, offset, count);
+        else if (pipeline_ == EPipeline::ForwardProjection)
+            ok = execute_fp(r);
+        else
+            ok = execute_iterative(r, done, total);
         if (!ok) return false;
-        if (pipeline_ == EPipeline::FDK) received_ += count;
+        if (pipeline_ == EPipeline::FDK) {
+            received_ += count;
+            done = received_;
+        }
         const bool tail = has_bytes(r.struct_size, offsetof(SExecutionRequest, progress_user_data) +
                                                        sizeof(r.progress_user_data));
-        if (tail && r.progress_callback)
-            r.progress_callback(r.progress_user_data, pipeline_ == EPipeline::FDK ? received_ : views_, views_);
+        if (tail && r.progress_callback) r.progress_callback(r.progress_user_data, done, total);
         return true;
     }
 
@@ -322,6 +365,70 @@ private:
         return true;
     }
 
+    // SART / SIRT / OSSART / CGLS (YkSession.cpp:290-350 prepare, :940-1030
+    // executeIterative_): the volume holds the start and must be in device
+    // memory; SART uses the TIGRE-approximate weights, SIRT the exact ones,
+    // OSSART the exact ones in golden-ratio subset order, CGLS the
+    // robust-restart strategy; non_negative clamps at zero (not for CGLS).
+    // iteration_count > 0 overrides the outer iterations of SIRT and OSSART.
+    bool execute_iterative(const SExecutionRequest& r, int& done, int& total)
+    {
+        if (r.volume.location != EMemoryLocation::Device ||
+            (pipeline_ == EPipeline::CGLS && r.projection.location != EMemoryLocation::Device))
+            return fail(EApiErrorCode::CudaError,
+                        "iterative pipelines need a device volume (and CGLS a device projection)");
+        DeviceScope scope(device_);
+        if (!scope.ok()) return fail(EApiErrorCode::CudaError, "cannot select the session's device");
+        const size_t np = view_elems_ * size_t(views_);
+        const float* d_b = r.projection.data;
+        if (r.projection.location == EMemoryLocation::Host) {
+            float* s = device_buffer(d_proj_, proj_have_, np);
+            if (!s || hipMemcpy(s, r.projection.data, np * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+                return fail(EApiErrorCode::CudaError, "projection upload failed");
+            d_b = s;
+        }
+        const bool sirt_or_os = pipeline_ == EPipeline::SIRT || pipeline_ == EPipeline::OSSART;
+        const int iterations = sirt_or_os && r.iteration_count > 0 ? r.iteration_count : iterations_;
+        if (pipeline_ == EPipeline::CGLS) {
+            yk_cgls_desc d{};
+            d.struct_size = sizeof(d);
+            d.strategy = YK_CGLS_ROBUST_RESTART;
+            d.iterations = iterations;
+            d.epsilon = 1e-8f;
+            d.restart_on_divergence = 1;
+            d.forward_kind = forward_kind_;
+            d.back_kind = back_kind_;
+            d.check_interval = 1;
+            d.patience = 1;
+            yk_cgls_stats st{};
+            st.struct_size = sizeof(st);
+            if (yk_status e = yk_cgls_ex(proj_, &d, d_b, r.volume.data, &st, nullptr); e != YK_OK)
+                return fail(EApiErrorCode::CudaError, std::string("CGLS: ") + yk_last_error());
+            done = st.completed_iterations;
+        } else {
+            yk_algebraic_desc d{};
+            d.struct_size = sizeof(d);
+            d.method = pipeline_ == EPipeline::SART ? YK_ALG_SART : pipeline_ == EPipeline::SIRT ? YK_ALG_SIRT : YK_ALG_OSSART;
+            d.weight_model = pipeline_ == EPipeline::SART ? YK_WEIGHTS_TIGRE : YK_WEIGHTS_DETAILED;
+            d.subset_order = pipeline_ == EPipeline::OSSART ? YK_ORDER_GOLDEN : YK_ORDER_SEQUENTIAL;
+            d.iterations = iterations;
+            d.subsets = subsets_;
+            d.relaxation = relaxation_;
+            d.relaxation_reduction = 1.f;
+            d.epsilon = 1e-6f;
+            d.use_min = non_negative_ ? 1 : 0;
+            d.min_value = 0.f;
+            d.forward_kind = forward_kind_;
+            d.back_kind = back_kind_;
+            d.tv_strength_reduction = 1.f;
+            if (yk_status e = yk_algebraic(proj_, &d, d_b, r.volume.data); e != YK_OK)
+                return fail(EApiErrorCode::CudaError, std::string("algebraic reconstruction: ") + yk_last_error());
+            done = iterations;
+        }
+        total = iterations;
+        return true;
+    }
+
     void release_handles()
     {
         if (fdk_) (void)yk_fdk_destroy(fdk_);
@@ -360,6 +467,10 @@ private:
     yk_projector* proj_ = nullptr;
     EPipeline pipeline_ = EPipeline::FDK;
     int forward_kind_ = YK_FP_SIDDON;
+    int back_kind_ = YK_BP_FDK_PSEUDO_MATCHED;
+    int iterations_ = 0, subsets_ = 1;
+    float relaxation_ = 1.f;
+    bool non_negative_ = true;
     int device_ = 0;
     int views_ = 0;
     int received_ = 0;

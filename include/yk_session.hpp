@@ -10,11 +10,16 @@
 //     YkGeometryBuilderTypes.hpp:11-117 (SSystemConfig and its scan, detector,
 //     volume and pose records),
 //   include/YKCBCT/algorithms/analytic/YkFdkParams.hpp, YkAnalyticFilterParams.hpp,
-//   include/YKCBCT/algorithms/projection/YkProjectionParams.hpp.
-// Field names, order, defaults and meanings follow those headers; records of
-// pipelines this library does not provide (helical, cylindrical, iterative
-// parameter blocks, statistical and plugin tails) are omitted, and such
-// requests are refused with UnsupportedCombination.  float3 is HIP's.
+//   YkWfbpParams.hpp, include/YKCBCT/algorithms/projection/YkProjectionParams.hpp,
+//   include/YKCBCT/algorithms/iterative/YkIterationParams.hpp, YkCglsParams.hpp,
+//   YkPwlsParams.hpp, YkTigreParams.hpp.
+// Field names, order, defaults and meanings follow those headers.  Pipelines:
+// FDK, ForwardProjection, SART, SIRT, OSSART and CGLS on flat detectors and
+// circular scans.  Records of what this library does not provide (helical
+// and cylindrical geometry, statistical and plugin tails) are omitted; the
+// PWLS, WFBP and TIGRE-gradient blocks are kept so code that fills them
+// compiles, and those pipelines are refused with UnsupportedCombination.
+// float3 is HIP's.
 //
 // The implementation (ReconstructionSessionFactory in libyk_hip.so) sits on
 // the C ABI of yk_hip.h.  Like the reference, a session is not thread-safe,
@@ -120,17 +125,74 @@ struct SFdkAlgoParams {
     SAnalyticFilterParams filter{};
 };
 
+// SART / SIRT / OSSART / CGLS: outer iterations, relaxation, OS-SART subsets
+struct SIterAlgoParams {
+    int iterations = 10;
+    float relaxation = 1.f;
+    int subsets = 1;
+};
+
+// convergence monitor of the cylindrical CGLS (not read on flat scans)
+struct SCglsAlgoParams {
+    int minimum_iterations = 0;
+    int check_interval = 1;
+    int patience = 1;
+    float relative_residual = 0.f;
+    bool robust_restart = true;
+};
+
+enum class EPwlsRegularizerSpec : int32_t { None, Quadratic, Huber };
+struct SPwlsAlgoParams {
+    EPwlsRegularizerSpec regularizer = EPwlsRegularizerSpec::Quadratic;
+    float regularization = 1e-3f;
+    float huber_delta = 3e-3f;
+    float epsilon = 1e-6f;
+    float lower_bound = 0.f;
+    float upper_bound = 3.40282347e+38f;
+    bool use_projection_weights = false;
+};
+
+enum class ETigreGradientMethodSpec : int32_t {
+    Sart, OsSart, Sirt, AsdPocs, OsAsdPocs, BAsdPocsBeta, Pcsd, OsPcsd, AwPcsd, OsAwPcsd, AwAsdPocs, OsAwAsdPocs,
+};
+// Of this block the algebraic pipelines read non_negative (clamp at 0).
+struct STigreGradientAlgoParams {
+    ETigreGradientMethodSpec method = ETigreGradientMethodSpec::OsSart;
+    int block_size = 20;
+    float lambda_reduction = 0.f;
+    bool nesterov_relaxation = false;
+    bool fdk_initialization = false;
+    bool non_negative = true;
+    int tv_iterations = 20;
+    float tv_alpha = 0.002f;
+    float tv_alpha_reduction = 0.95f;
+    float maximum_update_ratio = 0.95f;
+    float max_l2_error = -1.f;
+};
+
+struct SWfbpAlgoParams {
+    float redundancy_flat = 0.6f;
+    SAnalyticFilterParams filter{};
+};
+
 struct SReconstructionSpec {
     EPipeline pipeline = EPipeline::FDK;
     EForwardProjector forward_projector = EForwardProjector::Automatic;
     EBackProjector back_projector = EBackProjector::Automatic;
     SFdkAlgoParams fdk{};
+    SIterAlgoParams iterative{};
+    SCglsAlgoParams cgls{};
+    SPwlsAlgoParams pwls{};
+    STigreGradientAlgoParams tigre{};
+    SWfbpAlgoParams wfbp{};
 };
 
 // --- session ----------------------------------------------------------------
 enum class EMemoryLocation : int32_t { Host, Device };
 
 // Contiguous float arrays: projection [view][v][u], volume [z][y][x].
+// The iterative pipelines start from the volume's contents and need it in
+// device memory (CGLS the projection too).
 struct Buffer {
     float* data = nullptr;
     EMemoryLocation location = EMemoryLocation::Device;

@@ -4,7 +4,12 @@
 //   codes, forward projection, FDK offline vs streaming, streaming state
 //   machine, reset / reuse) and dlltest/test_shepp_logan.cpp:32-158 (DLL FP
 //   then DLL FDK of a modified 3-D Shepp-Logan, 47-view streaming;
-//   cosine > 0.85, NRMSE < 0.65, streaming max difference < 1e-5).
+//   cosine > 0.85, NRMSE < 0.65, streaming max difference < 1e-5), plus the
+//   iterative ones (:150-168 pose ignored outside forward projection,
+//   :207-244 SIRT smoke, :246-316 SIRT reset / reuse) and checks that each
+//   iterative pipeline runs the C-ABI solver with the reference's mapping
+//   (src/interface/YkSession.cpp:290-350, 940-1030,
+//   src/interface/YkProjectorSelection.hpp:11-44).
 // Prints one "PASS <name>" / "FAIL <name>: <why>" line per scenario and
 // exits non-zero if any failed.  --host-only runs the scenarios that are
 // decided before any device work (they need no GPU).
@@ -19,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "yk_hip.h"
 #include "yk_session.hpp"
 
 namespace {
@@ -165,7 +171,11 @@ Result outside_this_path()
     spec.geometry.detector = YK::EDetectorKind::Cylindrical;
     r = expect_error(*s, s->initialize(spec), YK::EApiErrorCode::UnsupportedCombination);
     if (!r.ok) return r;
-    spec = small_system(YK::EPipeline::SIRT, 32);
+    spec = small_system(YK::EPipeline::PWLS, 32);
+    r = expect_error(*s, s->initialize(spec), YK::EApiErrorCode::UnsupportedCombination);
+    if (!r.ok) return r;
+    spec = small_system(YK::EPipeline::SART, 32);
+    spec.reconstruction.back_projector = YK::EBackProjector::SeparableFootprint;
     r = expect_error(*s, s->initialize(spec), YK::EApiErrorCode::UnsupportedCombination);
     if (!r.ok) return r;
     spec = small_system(YK::EPipeline::ForwardProjection, 2);
@@ -478,6 +488,232 @@ Result forward_projection_pose_quarter_turn()
     return { true, "" };
 }
 
+// ---------------------------------------------------------------------------
+// iterative pipelines
+// ---------------------------------------------------------------------------
+YK::SSystemSpec iterative_system(YK::EPipeline pipeline, int views, int iterations, float relaxation, int subsets)
+{
+    auto spec = small_system(pipeline, views);
+    spec.geometry.circular.sid_mm = 80.f;
+    spec.geometry.circular.sdd_mm = 160.f;
+    spec.reconstruction.iterative.iterations = iterations;
+    spec.reconstruction.iterative.relaxation = relaxation;
+    spec.reconstruction.iterative.subsets = subsets;
+    return spec;
+}
+
+// dlltest/test_Task_dll.cpp:150-168: the forward-projection pose is ignored
+// by other pipelines, and a NaN pose is still an InvalidGeometry
+Result iterative_pose_ignored()
+{
+    auto s = make_session();
+    auto spec = iterative_system(YK::EPipeline::SIRT, 32, 1, 0.2f, 1);
+    spec.geometry.forward_projection_pose.enabled = true;
+    spec.geometry.forward_projection_pose.rotation_x_rad = 0.7f;
+    if (!s->initialize(spec)) return { false, s->lastErrorMessage() };
+    spec.geometry.forward_projection_pose.rotation_x_rad = NAN;
+    return expect_error(*s, s->initialize(spec), YK::EApiErrorCode::InvalidGeometry);
+}
+
+// dlltest/test_Task_dll.cpp:207-244: SIRT, one iteration, zero data
+Result sirt_zero_data()
+{
+    auto s = make_session();
+    auto spec = iterative_system(YK::EPipeline::SIRT, 32, 1, 0.2f, 1);
+    if (!s->initialize(spec)) return { false, s->lastErrorMessage() };
+    DeviceBuffer vol(size_t(kNx) * kNy * kNz), proj(size_t(32) * kNu * kNv);
+    YK::SExecutionRequest r{};
+    r.view_count = 32;
+    r.volume = dev(vol);
+    r.projection = dev(proj);
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    for (float x : vol.download())
+        if (x != 0.f) return { false, "zero data gave a non-zero volume" };
+    return { true, "" };
+}
+
+// dlltest/test_Task_dll.cpp:246-316: SIRT twice with reset in between, from
+// a cleared volume each time, gives the same result (max difference 1e-5)
+Result sirt_reset_reuse()
+{
+    constexpr int views = 24;
+    auto s = make_session();
+    auto spec = iterative_system(YK::EPipeline::SIRT, views, 2, 0.2f, 1);
+    if (!s->initialize(spec)) return { false, s->lastErrorMessage() };
+    auto hp = synthetic_projection(views);
+    DeviceBuffer vol(size_t(kNx) * kNy * kNz), proj(hp.size());
+    proj.upload(hp);
+    YK::SExecutionRequest r{};
+    r.volume = dev(vol);
+    r.projection = dev(proj);
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    const auto first = vol.download();
+    s->reset();
+    (void)hipMemset(vol.p, 0, vol.n * sizeof(float));
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    const auto second = vol.download();
+    if (!finite_nonzero(first)) return { false, "empty reconstruction" };
+    const double d = max_abs_diff(first, second);
+    if (!(d <= 1e-5)) return { false, "reset changed the result by " + std::to_string(d) };
+    return { true, "" };
+}
+
+// Each pipeline equals the C-ABI solver called with the reference's mapping:
+// SART -> TIGRE weights, SIRT -> exact weights, OSSART -> exact weights in
+// golden-ratio order, CGLS -> robust restart; Automatic -> Siddon forward and
+// the FDK pseudo-matched back (CGLS: the Siddon transpose); non_negative -> a
+// lower bound of 0 (not for CGLS).  Bit-identical: the same code runs.
+Result iterative_mapping()
+{
+    constexpr int views = 12;
+    const size_t nvol = size_t(kNx) * kNy * kNz;
+    auto hp = synthetic_projection(views);
+    DeviceBuffer proj(hp.size());
+    proj.upload(hp);
+    yk_ctx* ctx = nullptr;
+    if (yk_ctx_create(0, &ctx) != YK_OK) return { false, "context" };
+    std::unique_ptr<yk_ctx, yk_status (*)(yk_ctx*)> ctx_guard(ctx, yk_ctx_destroy);
+    yk_system sys{};
+    sys.scan.total_views = views;
+    sys.scan.views_per_turn = views;
+    sys.scan.rotation_direction = 1;
+    sys.scan.sid_mm = 80.f;
+    sys.scan.sdd_mm = 160.f;
+    sys.detector.channels = kNu;
+    sys.detector.rows = kNv;
+    sys.detector.channel_size_mm = 1.f;
+    sys.detector.row_size_mm = 1.f;
+    sys.volume = { kNx, kNy, kNz, 1.f, 1.f, 1.f, { 0.f, 0.f, 0.f } };
+    std::vector<yk_view> vw(views);
+    int n = views;
+    if (yk_geometry_build(&sys, vw.data(), views, &n) != YK_OK) return { false, yk_last_error() };
+    yk_projector* pr = nullptr;
+    if (yk_projector_create(ctx, vw.data(), views, kNu, kNv, &sys.volume, 6.2831853f, &pr) != YK_OK)
+        return { false, yk_last_error() };
+    std::unique_ptr<yk_projector, yk_status (*)(yk_projector*)> pr_guard(pr, yk_projector_destroy);
+
+    struct Case {
+        YK::EPipeline pipeline;
+        YK::EBackProjector back;
+        int method, weights, order, bk;
+    };
+    const Case cases[] = {
+        { YK::EPipeline::SART, YK::EBackProjector::Automatic, YK_ALG_SART, YK_WEIGHTS_TIGRE, YK_ORDER_SEQUENTIAL,
+          YK_BP_FDK_PSEUDO_MATCHED },
+        { YK::EPipeline::SIRT, YK::EBackProjector::JosephV3, YK_ALG_SIRT, YK_WEIGHTS_DETAILED, YK_ORDER_SEQUENTIAL,
+          YK_BP_JOSEPH_VOXEL },
+        { YK::EPipeline::OSSART, YK::EBackProjector::SiddonRayDriven, YK_ALG_OSSART, YK_WEIGHTS_DETAILED,
+          YK_ORDER_GOLDEN, YK_BP_SIDDON_ADJOINT },
+        { YK::EPipeline::CGLS, YK::EBackProjector::Automatic, -1, 0, 0, YK_BP_SIDDON_ADJOINT },
+    };
+    for (const Case& c : cases) {
+        auto s = make_session();
+        auto spec = iterative_system(c.pipeline, views, 3, 0.7f, 4);
+        spec.reconstruction.back_projector = c.back;
+        if (!s->initialize(spec)) return { false, s->lastErrorMessage() };
+        DeviceBuffer a(nvol), b(nvol);
+        YK::SExecutionRequest r{};
+        r.volume = dev(a);
+        r.projection = dev(proj);
+        if (!s->execute(r)) return { false, s->lastErrorMessage() };
+        yk_status st;
+        if (c.method < 0) {
+            yk_cgls_desc d{};
+            d.struct_size = sizeof(d);
+            d.strategy = YK_CGLS_ROBUST_RESTART;
+            d.iterations = 3;
+            d.epsilon = 1e-8f;
+            d.restart_on_divergence = 1;
+            d.forward_kind = YK_FP_SIDDON;
+            d.back_kind = c.bk;
+            d.check_interval = 1;
+            d.patience = 1;
+            st = yk_cgls_ex(pr, &d, proj.p, b.p, nullptr, nullptr);
+        } else {
+            yk_algebraic_desc d{};
+            d.struct_size = sizeof(d);
+            d.method = c.method;
+            d.weight_model = c.weights;
+            d.subset_order = c.order;
+            d.iterations = 3;
+            d.subsets = 4;
+            d.relaxation = 0.7f;
+            d.relaxation_reduction = 1.f;
+            d.epsilon = 1e-6f;
+            d.use_min = 1;
+            d.forward_kind = YK_FP_SIDDON;
+            d.back_kind = c.bk;
+            d.tv_strength_reduction = 1.f;
+            st = yk_algebraic(pr, &d, proj.p, b.p);
+        }
+        if (st != YK_OK) return { false, yk_last_error() };
+        const auto va = a.download(), vb = b.download();
+        if (!finite_nonzero(va)) return { false, "empty reconstruction" };
+        if (max_abs_diff(va, vb) != 0.0)
+            return { false, "pipeline " + std::to_string(int(c.pipeline)) + " differs from the C-ABI call" };
+    }
+    return { true, "" };
+}
+
+// buffer rules: device volume required (CudaError otherwise), CGLS also a
+// device projection; SART / SIRT / OSSART accept a host projection with the
+// same result; the whole sequence from view 0; iteration_count overrides the
+// SIRT / OSSART iterations; bad parameters fail initialisation
+Result iterative_requests()
+{
+    constexpr int views = 12;
+    const size_t nvol = size_t(kNx) * kNy * kNz;
+    auto hp = synthetic_projection(views);
+    DeviceBuffer proj(hp.size()), a(nvol), b(nvol);
+    proj.upload(hp);
+    std::vector<float> hv(nvol, 0.f);
+    auto s = make_session();
+    if (!s->initialize(iterative_system(YK::EPipeline::OSSART, views, 2, 0.5f, 3))) return { false, s->lastErrorMessage() };
+    YK::SExecutionRequest r{};
+    r.projection = dev(proj);
+    r.volume = host(hv);
+    Result e = expect_error(*s, s->execute(r), YK::EApiErrorCode::CudaError);
+    if (!e.ok) return e;
+    r.volume = dev(a);
+    r.view_offset = 1;
+    r.view_count = views - 1;
+    e = expect_error(*s, s->execute(r), YK::EApiErrorCode::StreamingStateError);
+    if (!e.ok) return e;
+    r.view_offset = 0;
+    r.view_count = 0;
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    r.projection = host(hp);
+    r.volume = dev(b);
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    if (max_abs_diff(a.download(), b.download()) != 0.0) return { false, "host projection changed the result" };
+    // iteration_count = 5 equals a session built with 5 iterations
+    (void)hipMemset(a.p, 0, nvol * sizeof(float));
+    (void)hipMemset(b.p, 0, nvol * sizeof(float));
+    r.projection = dev(proj);
+    r.volume = dev(a);
+    r.iteration_count = 5;
+    if (!s->execute(r)) return { false, s->lastErrorMessage() };
+    auto s5 = make_session();
+    if (!s5->initialize(iterative_system(YK::EPipeline::OSSART, views, 5, 0.5f, 3))) return { false, s5->lastErrorMessage() };
+    r.volume = dev(b);
+    r.iteration_count = 0;
+    if (!s5->execute(r)) return { false, s5->lastErrorMessage() };
+    if (max_abs_diff(a.download(), b.download()) != 0.0) return { false, "iteration_count did not override" };
+    // CGLS needs a device projection
+    auto c = make_session();
+    if (!c->initialize(iterative_system(YK::EPipeline::CGLS, views, 2, 1.f, 1))) return { false, c->lastErrorMessage() };
+    r.projection = host(hp);
+    e = expect_error(*c, c->execute(r), YK::EApiErrorCode::CudaError);
+    if (!e.ok) return e;
+    // bad parameters
+    auto bad = make_session();
+    e = expect_error(*bad, bad->initialize(iterative_system(YK::EPipeline::SIRT, views, 0, 0.5f, 1)),
+                     YK::EApiErrorCode::InvalidGeometry);
+    if (!e.ok) return e;
+    return expect_error(*bad, bad->initialize(iterative_system(YK::EPipeline::OSSART, views, 2, 0.5f, views + 1)),
+                        YK::EApiErrorCode::InvalidGeometry);
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -505,6 +741,11 @@ int main(int argc, char** argv)
         { "session_reset_and_reuse", session_reset_and_reuse, true },
         { "forward_projection_pose_quarter_turn", forward_projection_pose_quarter_turn, true },
         { "shepp_logan_fp_then_fdk", shepp_logan_fp_then_fdk, true },
+        { "iterative_pose_ignored", iterative_pose_ignored, true },
+        { "sirt_zero_data", sirt_zero_data, true },
+        { "sirt_reset_reuse", sirt_reset_reuse, true },
+        { "iterative_mapping", iterative_mapping, true },
+        { "iterative_requests", iterative_requests, true },
     };
     int failed = 0;
     for (const Case& c : cases) {

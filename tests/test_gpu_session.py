@@ -2,8 +2,10 @@
 DLL scenarios ported in tests/session_check.cpp (forward projection with each
 projector and with host / device buffers, execution validation, FDK offline
 vs 13 + 19-view streaming, the streaming state machine and completion-time
-host output, reset / release / reuse, the forward-projection pose, and the
-DLL Shepp-Logan FP -> FDK quality gates with 47-view streaming)."""
+host output, reset / release / reuse, the forward-projection pose, the
+DLL Shepp-Logan FP -> FDK quality gates with 47-view streaming) and the
+iterative pipelines (SIRT smoke and reset / reuse, the SART / SIRT / OSSART /
+CGLS mapping onto the C-ABI solvers, buffer and request rules)."""
 import subprocess
 
 import pytest
@@ -18,4 +20,4 @@ def test_session_scenarios():
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "FAIL" not in res.stdout
-    assert res.stdout.count("PASS") == 16
+    assert res.stdout.count("PASS") == 21
