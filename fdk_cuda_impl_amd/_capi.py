@@ -96,6 +96,27 @@ class AlgebraicDesc(ctypes.Structure):
                 ("tv_epsilon", ctypes.c_float), ("tv_strength_reduction", ctypes.c_float)]
 
 
+class FdkStage(ctypes.Structure):
+    _fields_ = [("first_view", ctypes.c_int), ("count", ctypes.c_int), ("d_input", ctypes.c_void_p),
+                ("d_preweighted", ctypes.c_void_p), ("d_filtered", ctypes.c_void_p),
+                ("filtered_storage", ctypes.c_int), ("d_row_scales", ctypes.c_void_p),
+                ("stream", ctypes.c_void_p)]
+
+
+_DIAG_FLOATS = ["min_source_to_axis_mm", "max_source_to_axis_mm", "min_source_to_detector_mm",
+                "max_source_to_detector_mm", "min_source_to_plane_mm", "max_source_to_plane_mm",
+                "max_abs_source_axis_mm", "max_detector_skew", "max_abs_principal_u_pix",
+                "max_abs_principal_v_pix", "min_principal_u_pix", "max_principal_u_pix",
+                "min_principal_v_pix", "max_principal_v_pix", "min_relative_tilt_u_rad",
+                "max_relative_tilt_u_rad", "min_relative_tilt_v_rad", "max_relative_tilt_v_rad",
+                "min_relative_tilt_n_rad", "max_relative_tilt_n_rad", "max_normal_misalignment"]
+
+
+class GeometryDiagnostics(ctypes.Structure):
+    _fields_ = ([("struct_size", ctypes.c_uint32)] + [(n, ctypes.c_float) for n in _DIAG_FLOATS]
+                + [("approximate", ctypes.c_int)])
+
+
 class CglsDesc(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("strategy", ctypes.c_int),
                 ("iterations", ctypes.c_int), ("epsilon", ctypes.c_float),
@@ -165,6 +186,17 @@ SIGNATURES = {
     "yk_backproject": (_I, [_P, _I, _P, _P, _I, _I, _I]),
     "yk_cgls": (_I, [_P, _I, _I, _P, _P, _I, _DP]),
     "yk_cgls_ex": (_I, [_P, ctypes.POINTER(CglsDesc), _P, _P, ctypes.POINTER(CglsStats), _DP]),
+    "yk_fdk_set_stage_capture": (_I, [_P, _I]),
+    "yk_fdk_last_stage": (_I, [_P, ctypes.POINTER(FdkStage)]),
+    "yk_fdk_geometry_diagnostics": (_I, [_P, ctypes.POINTER(GeometryDiagnostics)]),
+    "yk_geometry_diagnose": (_I, [_VP, _I, _I, _I, ctypes.POINTER(GeometryDiagnostics)]),
+    "yk_dump_create": (_I, [ctypes.c_char_p, _I, ctypes.POINTER(_P)]),
+    "yk_dump_destroy": (_I, [_P]),
+    "yk_dump_set_enabled": (_I, [_P, _I]),
+    "yk_dump_device_f32": (_I, [_P, ctypes.c_char_p, _P, _SZ, _P]),
+    "yk_dump_device_f32_a": (_I, [_P, ctypes.c_char_p, _I, _P, _SZ, _P]),
+    "yk_dump_text": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p]),
+    "yk_dump_text_a": (_I, [_P, ctypes.c_char_p, _I, ctypes.c_char_p]),
     "yk_sart": (_I, [_P, _I, _I, _P, _P, _I, _F, _I]),
     "yk_algebraic": (_I, [_P, ctypes.POINTER(AlgebraicDesc), _P, _P]),
     "yk_tv_gradient": (_I, [_P, ctypes.POINTER(VolumeGrid), _P, _F, _P, _DP]),

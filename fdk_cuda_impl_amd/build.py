@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "libyk_hip.so")
 SESSION_CHECK = os.path.join(HERE, "yk_session_check")
 ARCH = os.environ.get("YK_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["capi.cpp", "geometry.cpp", "fdk.cpp", "projector.cpp", "algebraic.cpp", "cgls.cpp", "session.cpp",
+SOURCES = ["capi.cpp", "geometry.cpp", "fdk.cpp", "projector.cpp", "algebraic.cpp", "cgls.cpp", "diagnostics.cpp", "session.cpp",
            "kernels_filter.hip", "kernels_bp.hip", "kernels_proj.hip", "kernels_misc.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -94,6 +94,22 @@ def _views_ptr(views: np.ndarray):
     return ctypes.cast(views.ctypes.data, ctypes.POINTER(C.View))
 
 
+def _diagnostics_dict(d) -> dict:
+    out = {name: float(getattr(d, name)) for name in C._DIAG_FLOATS}
+    out["approximate"] = bool(d.approximate)
+    return out
+
+
+def geometry_diagnostics(views: np.ndarray, nu: int, nv: int) -> dict:
+    """Calibrated-geometry report of a view sequence (host only; raises
+    YkError for geometry FDK cannot use, e.g. repeated adjacent angles)."""
+    views = np.ascontiguousarray(views, dtype=np.float32)
+    d = C.GeometryDiagnostics()
+    d.struct_size = ctypes.sizeof(d)
+    C.call("yk_geometry_diagnose", _views_ptr(views), views.shape[0], nu, nv, ctypes.byref(d))
+    return _diagnostics_dict(d)
+
+
 def grid_of(volume, voxel, center=(0.0, 0.0, 0.0)) -> C.VolumeGrid:
     g = C.VolumeGrid()
     g.nx, g.ny, g.nz = volume
@@ -312,6 +328,26 @@ class FdkPlan:
         out = np.zeros((self.n_views, 17), dtype=np.float32)
         C.call("yk_fdk_get_coefficients", self.handle, out.ctypes.data_as(C._FP))
         return out
+
+    def geometry_diagnostics(self) -> dict:
+        """How far the plan's per-view geometry is from the ideal circular scan
+        (FdkGeometryDiagnostics, YkFdkPipeline.hpp:35-58)."""
+        d = C.GeometryDiagnostics()
+        d.struct_size = ctypes.sizeof(d)
+        C.call("yk_fdk_geometry_diagnostics", self.handle, ctypes.byref(d))
+        return _diagnostics_dict(d)
+
+    def set_stage_capture(self, on: bool = True):
+        """Keep the weighted (pre-filter) projections of each execute chunk."""
+        C.call("yk_fdk_set_stage_capture", self.handle, 1 if on else 0)
+
+    def last_stage(self) -> dict:
+        """Device pointers of the last chunk yk_fdk_execute processed (input,
+        weighted when captured, filtered, row scales; FdkStageView,
+        YkFdkPipeline.hpp:176-191), valid until the next execute / reset."""
+        st = C.FdkStage()
+        C.call("yk_fdk_last_stage", self.handle, ctypes.byref(st))
+        return {name: getattr(st, name) for name, _ in C.FdkStage._fields_}
 
     def filter(self, proj, first: int = 0, storage: str | None = None, out=None, scales=None):
         """Preweight [+ Parker] + ramp filter of device views [first, first + len(proj))."""

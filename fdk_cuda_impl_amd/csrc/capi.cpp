@@ -288,6 +288,104 @@ yk_status yk_fdk_reset(yk_fdk* fdk)
     return guard([&] {
         require(fdk != nullptr, YK_ERR_INVALID_CONFIG, "null FDK plan");
         fdk->received = 0;
+        fdk->last_stage = yk_fdk_stage{};
+    });
+}
+
+yk_status yk_fdk_set_stage_capture(yk_fdk* fdk, int on)
+{
+    return guard([&] {
+        require(fdk != nullptr, YK_ERR_INVALID_CONFIG, "null FDK plan");
+        fdk->capture_stages = on != 0;
+    });
+}
+
+yk_status yk_fdk_last_stage(const yk_fdk* fdk, yk_fdk_stage* out)
+{
+    return guard([&] {
+        require(fdk != nullptr, YK_ERR_INVALID_CONFIG, "null FDK plan");
+        require(out != nullptr, YK_ERR_INVALID_BUFFER, "null output");
+        *out = fdk->last_stage;
+    });
+}
+
+yk_status yk_fdk_geometry_diagnostics(const yk_fdk* fdk, yk_geometry_diagnostics* out)
+{
+    return guard([&] {
+        require(fdk != nullptr, YK_ERR_INVALID_CONFIG, "null FDK plan");
+        require(out != nullptr && out->struct_size >= sizeof(yk_geometry_diagnostics), YK_ERR_INVALID_BUFFER,
+            "null or truncated yk_geometry_diagnostics");
+        *out = fdk->diagnostics;
+    });
+}
+
+yk_status yk_geometry_diagnose(const yk_view* views, int n_views, int nu, int nv, yk_geometry_diagnostics* out)
+{
+    return guard([&] {
+        require(views != nullptr && n_views > 0, YK_ERR_INVALID_BUFFER, "null geometry buffer");
+        require(out != nullptr && out->struct_size >= sizeof(yk_geometry_diagnostics), YK_ERR_INVALID_BUFFER,
+            "null or truncated yk_geometry_diagnostics");
+        std::vector<yk_view> v(views, views + n_views);
+        std::vector<DerivedD> d;
+        derive_geometry(v, nu, nv, 6.28318530717958647692, d);
+        geometry_diagnostics(v, d, *out);
+    });
+}
+
+yk_status yk_dump_create(const char* output_dir, int enabled, yk_dump** out)
+{
+    return guard([&] {
+        require(out != nullptr, YK_ERR_INVALID_BUFFER, "null output handle");
+        *out = nullptr;
+        *out = dump_create(output_dir, enabled);
+    });
+}
+
+yk_status yk_dump_destroy(yk_dump* dump)
+{
+    return guard([&] { dump_destroy(dump); });
+}
+
+yk_status yk_dump_set_enabled(yk_dump* dump, int enabled)
+{
+    return guard([&] {
+        require(dump != nullptr, YK_ERR_INVALID_CONFIG, "null dump");
+        dump_set_enabled(dump, enabled);
+    });
+}
+
+yk_status yk_dump_device_f32(yk_dump* dump, const char* file_name, const float* d_src, size_t count, void* stream)
+{
+    return guard([&] {
+        require(dump != nullptr, YK_ERR_INVALID_CONFIG, "null dump");
+        dump_device_f32(dump, file_name ? file_name : "", d_src, count, static_cast<hipStream_t>(stream));
+    });
+}
+
+yk_status yk_dump_device_f32_a(yk_dump* dump, const char* prefix, int index, const float* d_src, size_t count,
+    void* stream)
+{
+    return guard([&] {
+        require(dump != nullptr, YK_ERR_INVALID_CONFIG, "null dump");
+        require(prefix != nullptr, YK_ERR_INVALID_BUFFER, "null prefix");
+        dump_device_f32(dump, dump_name_a(prefix, index, ".raw"), d_src, count, static_cast<hipStream_t>(stream));
+    });
+}
+
+yk_status yk_dump_text(yk_dump* dump, const char* file_name, const char* text)
+{
+    return guard([&] {
+        require(dump != nullptr, YK_ERR_INVALID_CONFIG, "null dump");
+        dump_text(dump, file_name ? file_name : "", text);
+    });
+}
+
+yk_status yk_dump_text_a(yk_dump* dump, const char* prefix, int index, const char* text)
+{
+    return guard([&] {
+        require(dump != nullptr, YK_ERR_INVALID_CONFIG, "null dump");
+        require(prefix != nullptr, YK_ERR_INVALID_BUFFER, "null prefix");
+        dump_text(dump, dump_name_a(prefix, index, ".txt"), text);
     });
 }
 

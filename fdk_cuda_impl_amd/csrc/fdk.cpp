@@ -109,6 +109,7 @@ yk_fdk::yk_fdk(yk_ctx* c, const yk_fdk_desc& d, const yk_view* v, int n, int nu_
     ctx->activate();
     hipStream_t s = ctx->stream;
     derive_geometry(views, nu, nv, scan_range, derived);
+    geometry_diagnostics(views, derived, diagnostics);
     fdk_coefficients(views, derived, coeffs);
 
     // resolveParkerEnabled (include/YKCBCT/interface/YkSystemReconstruction.hpp:175-185)
@@ -210,7 +211,7 @@ void yk_fdk::build_tiles()
 }
 
 void yk_fdk::filter(const float* d_proj, int first, int count, void* d_out, int storage,
-    float* d_scales)
+    float* d_scales, float* d_preweighted)
 {
     if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
     if (!d_proj || !d_out) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
@@ -227,6 +228,7 @@ void yk_fdk::filter(const float* d_proj, int first, int count, void* d_out, int 
     p.nu = nu; p.nv = nv; p.count = count; p.n = n_pad;
     p.storage = storage;
     launch_fdk_filter(p, ctx->stream);
+    if (d_preweighted) launch_preweight(p, d_preweighted, ctx->stream);
 }
 
 void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_scales, int first,
@@ -293,6 +295,8 @@ void yk_fdk::execute(const float* projection, int ploc, size_t pcap, int offset,
     d_filtered_ws.resize(size_t(chunk) * view_elems * eb);
     if (desc.storage != YK_F32) d_scales_ws.resize(size_t(chunk) * nv);
     if (ploc == YK_HOST) d_proj_ws.resize(size_t(chunk) * view_elems);
+    if (capture_stages) d_preweighted_ws.resize(size_t(chunk) * view_elems);
+    last_stage = yk_fdk_stage{};
     for (int base = 0; base < count; base += chunk) {
         const int k = std::min(chunk, count - base);
         const float* d_proj = projection + size_t(base) * view_elems;
@@ -302,7 +306,16 @@ void yk_fdk::execute(const float* projection, int ploc, size_t pcap, int offset,
             d_proj = d_proj_ws.get();
         }
         filter(d_proj, offset + base, k, d_filtered_ws.get(), desc.storage,
-            desc.storage == YK_F32 ? nullptr : d_scales_ws.get());
+            desc.storage == YK_F32 ? nullptr : d_scales_ws.get(),
+            capture_stages ? d_preweighted_ws.get() : nullptr);
+        last_stage.first_view = offset + base;
+        last_stage.count = k;
+        last_stage.d_input = d_proj;
+        last_stage.d_preweighted = capture_stages ? d_preweighted_ws.get() : nullptr;
+        last_stage.d_filtered = d_filtered_ws.get();
+        last_stage.filtered_storage = desc.storage;
+        last_stage.d_row_scales = desc.storage == YK_F32 ? nullptr : d_scales_ws.get();
+        last_stage.stream = s;
         const int acc = (clear_output && base == 0) ? 0 : 1;
         backproject(d_filtered_ws.get(), desc.storage,
             desc.storage == YK_F32 ? nullptr : d_scales_ws.get(), offset + base, k, d_vol, acc);

@@ -1,6 +1,7 @@
 // FDK plan and projection-operator plan (library-owned handles of the C ABI).
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "yk_internal.hpp"
@@ -9,7 +10,8 @@ struct yk_fdk {
     yk_fdk(yk_ctx* ctx, const yk_fdk_desc& desc, const yk_view* views, int n_views, int nu,
         int nv, const yk_volume_grid& grid);
 
-    void filter(const float* d_proj, int first, int count, void* d_out, int storage, float* d_scales);
+    void filter(const float* d_proj, int first, int count, void* d_out, int storage, float* d_scales,
+        float* d_preweighted = nullptr);
     void backproject(const void* d_filtered, int storage, const float* d_scales, int first, int count,
         float* d_volume, int accumulate);
     void execute(const float* projection, int ploc, size_t pcap, int offset, int count, float* volume,
@@ -48,6 +50,12 @@ struct yk_fdk {
     yk::DeviceArray<float> d_scales_ws;
     yk::DeviceArray<float> d_volume_ws;
     int received = 0;
+
+    // calibrated-geometry report (diagnostics.cpp) and the last chunk's stages
+    yk_geometry_diagnostics diagnostics{};
+    bool capture_stages = false;
+    yk::DeviceArray<float> d_preweighted_ws;
+    yk_fdk_stage last_stage{};
 
 private:
     void build_tiles();
@@ -98,3 +106,15 @@ void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b
 // CGLS, robust-restart and ASTRA strategies (cgls.cpp); throws through fail()
 void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d_x, yk_cgls_stats* stats,
     double* residual_norms);
+
+// calibrated-geometry diagnostics (diagnostics.cpp); throws INVALID_GEOMETRY
+// for geometry FDK cannot use
+void geometry_diagnostics(const std::vector<yk_view>& views, const std::vector<yk::DerivedD>& derived,
+    yk_geometry_diagnostics& out);
+// raw dumps (diagnostics.cpp)
+std::string dump_name_a(const char* prefix, int a, const char* ext);
+yk_dump* dump_create(const char* dir, int enabled);
+void dump_destroy(yk_dump* d);
+void dump_set_enabled(yk_dump* d, int on);
+void dump_device_f32(yk_dump* d, const std::string& name, const float* src, size_t count, hipStream_t s);
+void dump_text(yk_dump* d, const std::string& name, const char* text);

@@ -319,7 +319,27 @@ void launch_storage(const FilterLaunch& p, int rows, size_t lds, hipStream_t s)
     }
 }
 
+// weighted projections alone (the stage the filter consumes): out = p * w
+__global__ void __launch_bounds__(256)
+preweight_kernel(const float* __restrict__ proj, float* __restrict__ out, const FilterView* __restrict__ views,
+    const float* __restrict__ parker, int nu, int nv)
+{
+    const int u = blockIdx.x * 256 + int(threadIdx.x);
+    const int v = blockIdx.y, i = blockIdx.z;
+    if (u >= nu) return;
+    const size_t at = ((size_t)i * nv + v) * nu + u;
+    out[at] = proj[at] * row_weight(views[i], parker ? parker + (size_t)i * nu : nullptr, v, u);
+}
+
 }  // namespace
+
+void launch_preweight(const FilterLaunch& p, float* out, hipStream_t s)
+{
+    if (p.count <= 0) return;
+    dim3 grid((p.nu + 255) / 256, p.nv, p.count);
+    hipLaunchKernelGGL(preweight_kernel, grid, dim3(256), 0, s, p.proj, out, p.views, p.parker, p.nu, p.nv);
+    YK_HIP(hipGetLastError());
+}
 
 void launch_fdk_filter(const FilterLaunch& p, hipStream_t s)
 {

@@ -192,6 +192,8 @@ struct FilterLaunch {
     int storage;
 };
 void launch_fdk_filter(const FilterLaunch& p, hipStream_t s);
+// the cosine (and Parker) weighted projections of p's views into out ([count][nv][nu])
+void launch_preweight(const FilterLaunch& p, float* out, hipStream_t s);
 
 struct TileLaunch {
     const double* coeff64;      // device, per view: cu[4] cv[4] cd[4] (12 doubles)

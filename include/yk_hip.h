@@ -248,6 +248,59 @@ yk_status yk_fdk_execute(yk_fdk* fdk, const float* projection, int projection_lo
     int volume_location, size_t volume_capacity, int clear_output);
 yk_status yk_fdk_reset(yk_fdk* fdk);
 int yk_fdk_received_views(const yk_fdk* fdk);
+
+/* Read-only device view of the last chunk yk_fdk_execute processed
+ * (FdkStageView / lastStage, YkFdkPipeline.hpp:176-191, 549): the input on the
+ * device (the caller's device buffer, or the library's staging copy of a host
+ * one), the cosine- (and Parker-) weighted projections when stage capture is
+ * on (else null), and the filtered projections the backprojection consumed,
+ * in the plan's storage with their per-row scales for F16 / BF16.  Valid
+ * until the next execute, reset or destroy; the work is ordered on `stream`.
+ * No counterpart fields: filtered_storage, d_row_scales. */
+typedef struct yk_fdk_stage {
+    int first_view;
+    int count;                       /* 0: no chunk yet */
+    const float* d_input;            /* [count][nv][nu] */
+    const float* d_preweighted;      /* [count][nv][nu] or null */
+    const void* d_filtered;          /* [count][nv][nu] in filtered_storage */
+    int filtered_storage;            /* yk_storage */
+    const float* d_row_scales;       /* [count][nv] for F16 / BF16, else null */
+    void* stream;                    /* hipStream_t */
+} yk_fdk_stage;
+/* on: yk_fdk_execute also writes the weighted projections of each chunk
+ * (one more pass over the chunk; off by default) */
+yk_status yk_fdk_set_stage_capture(yk_fdk* fdk, int on);
+yk_status yk_fdk_last_stage(const yk_fdk* fdk, yk_fdk_stage* out);
+
+/* Distance of calibrated per-view geometry from the ideal circular scan
+ * (FdkGeometryDiagnostics, YkFdkPipeline.hpp:35-58, computed as
+ * validateApproximateFdkGeometry_, :664-866): source-to-axis, source-to-
+ * detector (radial) and source-to-plane ranges, the largest |source z|,
+ * detector skew |U.V| / (|U||V|), principal-point ranges in pixels, detector
+ * tilts about U, V, N relative to the ideal detector of each source, the
+ * largest 1 - |radial . normal|, and `approximate` when any of them (or a
+ * pixel pitch) departs by more than 1e-3 relative.  Geometry FDK cannot use
+ * (non-positive pitches or distances, non-finite angles, repeated adjacent
+ * angles) is INVALID_GEOMETRY, as in the plan constructors. */
+typedef struct yk_geometry_diagnostics {
+    uint32_t struct_size;            /* caller sets sizeof(yk_geometry_diagnostics) */
+    float min_source_to_axis_mm, max_source_to_axis_mm;
+    float min_source_to_detector_mm, max_source_to_detector_mm;
+    float min_source_to_plane_mm, max_source_to_plane_mm;
+    float max_abs_source_axis_mm;
+    float max_detector_skew;
+    float max_abs_principal_u_pix, max_abs_principal_v_pix;
+    float min_principal_u_pix, max_principal_u_pix;
+    float min_principal_v_pix, max_principal_v_pix;
+    float min_relative_tilt_u_rad, max_relative_tilt_u_rad;
+    float min_relative_tilt_v_rad, max_relative_tilt_v_rad;
+    float min_relative_tilt_n_rad, max_relative_tilt_n_rad;
+    float max_normal_misalignment;
+    int approximate;
+} yk_geometry_diagnostics;
+yk_status yk_geometry_diagnose(const yk_view* views, int n_views, int nu, int nv,
+    yk_geometry_diagnostics* out);
+yk_status yk_fdk_geometry_diagnostics(const yk_fdk* fdk, yk_geometry_diagnostics* out);
 int yk_fdk_complete(const yk_fdk* fdk);
 
 /* Stage entry points on device buffers.
@@ -399,6 +452,25 @@ yk_status yk_dot(yk_ctx* ctx, const float* d_a, const float* d_b, size_t n, doub
  * floats: centre[3], semi_axes[3], R[9] (body = R (p - centre)), value. */
 yk_status yk_phantom_ellipsoids(yk_ctx* ctx, const float* ellipsoids, int count,
     const yk_volume_grid* volume, int clamp_nonnegative, float* d_volume);
+
+/* ------------------------------------------------------------------------ */
+/* raw dumps (DumpManager, src/util/YkIoDump.hpp:9-72)                       */
+/* ------------------------------------------------------------------------ */
+/* Float32 device buffers and text into files under output_dir, little-endian
+ * with no header: projections [view][v][u], volumes [z][y][x].  The *_a forms
+ * name the file <prefix>_a<4-digit index>.raw / .txt.  A dump copies through
+ * a pinned host buffer after synchronizing `stream` (hipStream_t, null: the
+ * default stream).  While disabled every call succeeds without writing. */
+typedef struct yk_dump yk_dump;
+yk_status yk_dump_create(const char* output_dir, int enabled, yk_dump** out);
+yk_status yk_dump_destroy(yk_dump* dump);
+yk_status yk_dump_set_enabled(yk_dump* dump, int enabled);
+yk_status yk_dump_device_f32(yk_dump* dump, const char* file_name, const float* d_src,
+    size_t count, void* stream);
+yk_status yk_dump_device_f32_a(yk_dump* dump, const char* prefix, int index, const float* d_src,
+    size_t count, void* stream);
+yk_status yk_dump_text(yk_dump* dump, const char* file_name, const char* text);
+yk_status yk_dump_text_a(yk_dump* dump, const char* prefix, int index, const char* text);
 
 #ifdef __cplusplus
 }
