@@ -241,12 +241,22 @@ void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_sca
     const size_t quad = storage == YK_F32 ? 16 : 8;
     if (size_t(lds_quads) * quad > 160 * 1024) fail(YK_ERR_UNSUPPORTED, "footprint exceeds LDS");
     hipStream_t s = ctx->stream;
+    int kernel_storage = storage;
+    const void* data = d_filtered;
     if (storage != YK_F32) {
+        // one pass applies the normalized row factors, so the backprojection
+        // stages plain half texels (when nu is even: texel pairs)
         d_rowfac.resize(std::max(d_rowfac.size(), size_t(count) * nv));
         launch_row_factors(d_scales, count * nv, storage, d_maxes.get() + 4, d_rowfac.get(), s);
+        if (nu % 2 == 0) {
+            d_norm_ws.resize(std::max(d_norm_ws.size(), size_t(count) * nv * nu));
+            launch_normalize_rows(d_filtered, d_rowfac.get(), storage, count * nv, nu, d_norm_ws.get(), s);
+            data = d_norm_ws.get();
+            kernel_storage = kBpHalfNormalized;
+        }
     }
     BpLaunch p{};
-    p.proj = d_filtered;
+    p.proj = data;
     p.row_factors = storage == YK_F32 ? nullptr : d_rowfac.get();
     p.gscale_bits = d_maxes.get() + 4;
     p.coeffs = d_coeffs.get() + first;
@@ -257,7 +267,7 @@ void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_sca
     p.volume = d_volume;
     p.vol = vol;
     p.nu = nu; p.nv = nv; p.count = count;
-    p.storage = storage;
+    p.storage = kernel_storage;
     p.accumulate = accumulate;
     p.kahan = desc.kahan;
     p.general = general ? 1 : 0;

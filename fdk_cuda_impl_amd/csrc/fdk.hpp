@@ -41,6 +41,7 @@ struct yk_fdk {
     yk::DeviceArray<uint2> d_stage;
     yk::DeviceArray<int> d_maxes;      // [0..3] footprint maxima, [4] global row-scale bits
     yk::DeviceArray<unsigned> d_rowfac; // normalized row factors of the views being backprojected
+    yk::DeviceArray<unsigned short> d_norm_ws; // half texels times their row factors (half / bf16 storage)
     int nbx = 0, nby = 0, nbz = 0, bz = 0;
     int lds_quads = 0;
 

@@ -39,6 +39,11 @@
 namespace yk {
 namespace {
 
+// internal storage tag of the kernel: half texels already multiplied by their
+// normalized row factor (normalize_rows_kernel), so staging applies nothing
+constexpr int kHalfNorm = 16;
+constexpr bool has_factors(int storage) { return storage == YK_F16 || storage == YK_BF16; }
+
 constexpr int kBX = 16;
 constexpr int kBY = 16;
 constexpr int kThreads = kBX * kBY;
@@ -180,6 +185,31 @@ __global__ void row_factor_kernel(const float* __restrict__ s, int n, const int*
     }
 }
 
+// half / bfloat16 rows times their normalized row factor, as half texels:
+// the values the staging would otherwise form once per (block, view) pair
+// (exact for half: the factor is a power of two; rows below 2^-24 of the
+// largest row scale become 0 as in row_factor_kernel).  Pairs of texels, nu even.
+template <int STORAGE>
+__global__ void __launch_bounds__(256)
+normalize_rows_kernel(const unsigned* __restrict__ in, const unsigned* __restrict__ factors,
+    unsigned* __restrict__ out, int pairs_per_row, size_t n_pairs)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const unsigned f = factors[i / (size_t)pairs_per_row];
+        const unsigned raw = in[i];
+        half2_t h;
+        if constexpr (STORAGE == YK_F16) {
+            h = __builtin_bit_cast(half2_t, raw) * __builtin_bit_cast(half2_t, f);
+        } else {
+            const float ff = __uint_as_float(f);
+            h = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(
+                __uint_as_float(raw << 16) * ff, __uint_as_float(raw & 0xffff0000u) * ff));
+        }
+        out[i] = __builtin_bit_cast(unsigned, h);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // staging
 // ---------------------------------------------------------------------------
@@ -230,6 +260,8 @@ template <>
 struct Pair<YK_F16> { using T = unsigned; };
 template <>
 struct Pair<YK_BF16> { using T = unsigned; };
+template <>
+struct Pair<kHalfNorm> { using T = unsigned; };
 
 // texel pair (col, col + 1) of one row at byte offset `off`: float2, or the
 // two 16-bit texels as one dword (lo = col), read by a single dword load at a
@@ -260,7 +292,7 @@ __device__ __forceinline__ unsigned factor_offset(int row)
 template <int STORAGE>
 struct Prefetch {
     typename Pair<STORAGE>::T v[kMaxRows];
-    unsigned f[STORAGE == YK_F32 ? 1 : kMaxRows];
+    unsigned f[has_factors(STORAGE) ? kMaxRows : 1];
 };
 
 // Loads only: nothing consumes the loaded values before commit, so every load
@@ -279,7 +311,7 @@ __device__ __forceinline__ void prefetch_tile(Prefetch<STORAGE>& pf, const Stage
     for (int j = 0; j < kMaxRows; ++j) {
         if (j <= rows_per) {
             pf.v[j] = load_pair<STORAGE>(data, unsigned(off0 + j * rowb));
-            if constexpr (STORAGE != YK_F32)
+            if constexpr (has_factors(STORAGE))
                 pf.f[j] = __builtin_amdgcn_raw_buffer_load_b32(fac, factor_offset(row0 + j), 0, 0);
         }
     }
@@ -303,7 +335,10 @@ __device__ __forceinline__ typename Pair<STORAGE>::T shift_pair(typename Pair<ST
 template <int STORAGE>
 __device__ __forceinline__ half2_t to_half2(unsigned raw, unsigned f)
 {
-    if constexpr (STORAGE == YK_F16) {
+    if constexpr (STORAGE == kHalfNorm) {
+        (void)f;
+        return __builtin_bit_cast(half2_t, raw);
+    } else if constexpr (STORAGE == YK_F16) {
         return __builtin_bit_cast(half2_t, raw) * __builtin_bit_cast(half2_t, f);
     } else {
         const float ff = __uint_as_float(f);
@@ -340,7 +375,7 @@ __device__ __forceinline__ void commit_tile(void* lds, Prefetch<STORAGE>& pf, co
         half2_t x[kMaxRows];
 #pragma unroll
         for (int j = 0; j < kMaxRows; ++j)
-            if (j <= rows_per) x[j] = to_half2<STORAGE>(pf.v[j], pf.f[j]);
+            if (j <= rows_per) x[j] = to_half2<STORAGE>(pf.v[j], pf.f[has_factors(STORAGE) ? j : 0]);
 #pragma unroll
         for (int j = 0; j + 1 < kMaxRows; ++j) {
             if (j < rows_per) {
@@ -371,8 +406,14 @@ __device__ void stage_direct(void* lds, __amdgpu_buffer_rsrc_t data, __amdgpu_bu
         if constexpr (STORAGE == YK_F32) {
             static_cast<float4*>(lds)[slot] = make_float4(a.x, a.y, b.x - a.x, b.y - a.y);
         } else {
-            const half2_t xa = to_half2<STORAGE>(a, __builtin_amdgcn_raw_buffer_load_b32(fac, factor_offset(rt), 0, 0));
-            const half2_t xb = to_half2<STORAGE>(b, __builtin_amdgcn_raw_buffer_load_b32(fac, factor_offset(rt + 1), 0, 0));
+            half2_t xa, xb;
+            if constexpr (has_factors(STORAGE)) {
+                xa = to_half2<STORAGE>(a, __builtin_amdgcn_raw_buffer_load_b32(fac, factor_offset(rt), 0, 0));
+                xb = to_half2<STORAGE>(b, __builtin_amdgcn_raw_buffer_load_b32(fac, factor_offset(rt + 1), 0, 0));
+            } else {
+                xa = to_half2<STORAGE>(a, 0u);
+                xb = to_half2<STORAGE>(b, 0u);
+            }
             uint2 q;
             q.x = __builtin_bit_cast(unsigned, xa);
             q.y = __builtin_bit_cast(unsigned, xb - xa);
@@ -528,7 +569,7 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
         return make_rsrc(static_cast<const char*>(proj) + (size_t)view_bytes * i, view_bytes);
     };
     auto fac_rsrc = [&](int i) {
-        return make_rsrc(row_factors + (STORAGE == YK_F32 ? 0 : (size_t)nv * i), unsigned(nv) * 4u);
+        return make_rsrc(row_factors + (has_factors(STORAGE) ? (size_t)nv * i : 0), unsigned(nv) * 4u);
     };
 
     float g = 1.f;
@@ -689,18 +730,36 @@ void launch_row_factors(const float* scales, int n, int storage, int* gbits, uns
     YK_HIP(hipGetLastError());
 }
 
+void launch_normalize_rows(const void* in, const unsigned* factors, int storage, int rows, int nu, void* out,
+    hipStream_t s)
+{
+    if (nu % 2) fail(YK_ERR_UNSUPPORTED, "odd detector width");
+    const size_t pairs = size_t(rows) * (nu / 2);
+    const int threads = 256;
+    const unsigned blocks = unsigned(std::min<size_t>(8192, (pairs + threads - 1) / threads));
+    if (storage == YK_F16)
+        hipLaunchKernelGGL(normalize_rows_kernel<YK_F16>, dim3(blocks), dim3(threads), 0, s,
+            static_cast<const unsigned*>(in), factors, static_cast<unsigned*>(out), nu / 2, pairs);
+    else
+        hipLaunchKernelGGL(normalize_rows_kernel<YK_BF16>, dim3(blocks), dim3(threads), 0, s,
+            static_cast<const unsigned*>(in), factors, static_cast<unsigned*>(out), nu / 2, pairs);
+    YK_HIP(hipGetLastError());
+}
+
 void launch_fdk_bp(const BpLaunch& p, hipStream_t s)
 {
     // staging offsets are 32-bit byte offsets of rows in [-4400, nv + 4400)
     if ((size_t(p.nv) + 8800) * size_t(p.nu) * 4 >= (size_t(1) << 31))
         fail(YK_ERR_UNSUPPORTED, "detector too large for 32-bit staging offsets");
-    if (p.storage != YK_F32 && !p.row_factors) fail(YK_ERR_INVALID_BUFFER, "row factors missing");
+    if ((p.storage == YK_F16 || p.storage == YK_BF16) && !p.row_factors)
+        fail(YK_ERR_INVALID_BUFFER, "row factors missing");
     const int bz = p.bz;
     const int* gbits = p.gscale_bits;
     switch (p.storage) {
     case YK_F32: launch_storage<YK_F32>(p, gbits, s, bz); break;
     case YK_F16: launch_storage<YK_F16>(p, gbits, s, bz); break;
     case YK_BF16: launch_storage<YK_BF16>(p, gbits, s, bz); break;
+    case kBpHalfNormalized: launch_storage<kHalfNorm>(p, gbits, s, bz); break;
     default: fail(YK_ERR_INVALID_CONFIG, "unknown storage type");
     }
     YK_HIP(hipGetLastError());

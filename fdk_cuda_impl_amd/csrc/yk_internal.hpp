@@ -229,6 +229,11 @@ struct BpLaunch {
     int lds_quads;              // LDS capacity in quads
 };
 void launch_fdk_bp(const BpLaunch& p, hipStream_t s);
+// BpLaunch::storage of half texels already normalized by launch_normalize_rows
+constexpr int kBpHalfNormalized = 16;
+// rows x nu half / bfloat16 texels times their normalized row factors -> half
+void launch_normalize_rows(const void* in, const unsigned* factors, int storage, int rows, int nu, void* out,
+    hipStream_t s);
 // global maximum row scale into *gbits, then per row the factor scale / max
 // (packed half pair for half storage, float bits for bfloat16)
 void launch_row_factors(const float* scales, int n, int storage, int* gbits, unsigned* factors,
