@@ -47,7 +47,7 @@ constexpr bool has_factors(int storage) { return storage == YK_F16 || storage ==
 constexpr int kBX = 16;
 constexpr int kBY = 16;
 constexpr int kThreads = kBX * kBY;
-constexpr int kKahanViews = 16;
+constexpr int kKahanViews = 64;
 constexpr int kMaxRows = 12;  // texel rows a thread prefetches (quad rows + 1)
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));

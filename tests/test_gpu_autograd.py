@@ -73,7 +73,8 @@ def test_dot_identity_through_autograd(ctx):
     yt = torch.from_numpy(b).cuda().requires_grad_(True)
     lhs = torch.sum(ag.project(xt, P).double() * yt.double())
     rhs = torch.sum(xt.double() * ag.backproject(yt, P).double())
-    assert abs(float(lhs) - float(rhs)) <= 1e-5 * abs(float(lhs))
+    lhs, rhs = float(lhs.detach()), float(rhs.detach())
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
     # backward of the transpose is the forward model
     (gy,) = torch.autograd.grad(torch.sum(ag.backproject(yt, P) * xt.detach()), yt)
     assert rel_rmse(gy.cpu().numpy(), P.forward(xt.detach()).cpu().numpy()) < 1e-6
