@@ -2,6 +2,7 @@
 // footprint table, workspaces and the streaming execute contract of the
 // reference FdkPipeline / Session FDK branch.
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 
@@ -245,10 +246,10 @@ void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_sca
     const void* data = d_filtered;
     if (storage != YK_F32) {
         // one pass applies the normalized row factors, so the backprojection
-        // stages plain half texels (when nu is even: texel pairs)
+        // stages plain half texels (when nu is a multiple of 8: 16-byte rows)
         d_rowfac.resize(std::max(d_rowfac.size(), size_t(count) * nv));
         launch_row_factors(d_scales, count * nv, storage, d_maxes.get() + 4, d_rowfac.get(), s);
-        if (nu % 2 == 0) {
+        if (nu % 8 == 0 && reinterpret_cast<uintptr_t>(d_filtered) % 16 == 0) {
             d_norm_ws.resize(std::max(d_norm_ws.size(), size_t(count) * nv * nu));
             launch_normalize_rows(d_filtered, d_rowfac.get(), storage, count * nv, nu, d_norm_ws.get(), s);
             data = d_norm_ws.get();

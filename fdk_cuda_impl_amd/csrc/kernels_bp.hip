@@ -188,25 +188,35 @@ __global__ void row_factor_kernel(const float* __restrict__ s, int n, const int*
 // half / bfloat16 rows times their normalized row factor, as half texels:
 // the values the staging would otherwise form once per (block, view) pair
 // (exact for half: the factor is a power of two; rows below 2^-24 of the
-// largest row scale become 0 as in row_factor_kernel).  Pairs of texels, nu even.
+// largest row scale become 0 as in row_factor_kernel).  One workgroup per row
+// at a time, four texel pairs (16 bytes) per lane and step; nu % 8 == 0.
+template <int STORAGE>
+__device__ __forceinline__ unsigned normalize_pair(unsigned raw, unsigned f)
+{
+    half2_t h;
+    if constexpr (STORAGE == YK_F16) {
+        h = __builtin_bit_cast(half2_t, raw) * __builtin_bit_cast(half2_t, f);
+    } else {
+        const float ff = __uint_as_float(f);
+        h = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(
+            __uint_as_float(raw << 16) * ff, __uint_as_float(raw & 0xffff0000u) * ff));
+    }
+    return __builtin_bit_cast(unsigned, h);
+}
+
 template <int STORAGE>
 __global__ void __launch_bounds__(256)
-normalize_rows_kernel(const unsigned* __restrict__ in, const unsigned* __restrict__ factors,
-    unsigned* __restrict__ out, int pairs_per_row, size_t n_pairs)
+normalize_rows_kernel(const uint4* __restrict__ in, const unsigned* __restrict__ factors,
+    uint4* __restrict__ out, int quads_per_row, int rows)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const unsigned f = factors[i / (size_t)pairs_per_row];
-        const unsigned raw = in[i];
-        half2_t h;
-        if constexpr (STORAGE == YK_F16) {
-            h = __builtin_bit_cast(half2_t, raw) * __builtin_bit_cast(half2_t, f);
-        } else {
-            const float ff = __uint_as_float(f);
-            h = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(
-                __uint_as_float(raw << 16) * ff, __uint_as_float(raw & 0xffff0000u) * ff));
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const unsigned f = factors[r];
+        const size_t base = (size_t)r * quads_per_row;
+        for (int c = threadIdx.x; c < quads_per_row; c += blockDim.x) {
+            const uint4 v = in[base + c];
+            out[base + c] = make_uint4(normalize_pair<STORAGE>(v.x, f), normalize_pair<STORAGE>(v.y, f),
+                                       normalize_pair<STORAGE>(v.z, f), normalize_pair<STORAGE>(v.w, f));
         }
-        out[i] = __builtin_bit_cast(unsigned, h);
     }
 }
 
@@ -733,16 +743,16 @@ void launch_row_factors(const float* scales, int n, int storage, int* gbits, uns
 void launch_normalize_rows(const void* in, const unsigned* factors, int storage, int rows, int nu, void* out,
     hipStream_t s)
 {
-    if (nu % 2) fail(YK_ERR_UNSUPPORTED, "odd detector width");
-    const size_t pairs = size_t(rows) * (nu / 2);
-    const int threads = 256;
-    const unsigned blocks = unsigned(std::min<size_t>(8192, (pairs + threads - 1) / threads));
+    if (nu % 8) fail(YK_ERR_UNSUPPORTED, "detector width not a multiple of 8");
+    const int quads = nu / 8;  // 16-byte groups of 8 texels
+    const int threads = quads >= 256 ? 256 : (quads + 63) / 64 * 64;
+    const unsigned blocks = unsigned(std::min(rows, 16384));
     if (storage == YK_F16)
         hipLaunchKernelGGL(normalize_rows_kernel<YK_F16>, dim3(blocks), dim3(threads), 0, s,
-            static_cast<const unsigned*>(in), factors, static_cast<unsigned*>(out), nu / 2, pairs);
+            static_cast<const uint4*>(in), factors, static_cast<uint4*>(out), quads, rows);
     else
         hipLaunchKernelGGL(normalize_rows_kernel<YK_BF16>, dim3(blocks), dim3(threads), 0, s,
-            static_cast<const unsigned*>(in), factors, static_cast<unsigned*>(out), nu / 2, pairs);
+            static_cast<const uint4*>(in), factors, static_cast<uint4*>(out), quads, rows);
     YK_HIP(hipGetLastError());
 }
 

@@ -110,8 +110,12 @@ def shepp_logan_projection(ctx, g, kind="siddon"):
 
 @pytest.mark.parametrize("storage,tol", [("f16", 1e-3), ("bf16", 8e-3)])
 @pytest.mark.parametrize("kahan", [False, True])
-def test_reduced_precision_storage(ctx, oracle, storage, tol, kahan):
-    g = small_geometry(total_views=90, views_per_turn=90, volume=(32, 32, 24), voxel_mm=(0.8, 0.8, 0.8))
+@pytest.mark.parametrize("channels", [44, 48])
+def test_reduced_precision_storage(ctx, oracle, storage, tol, kahan, channels):
+    """44 channels: the backprojection applies the row factors while staging;
+    48 (a multiple of 8): one pass normalizes the rows first."""
+    g = small_geometry(total_views=90, views_per_turn=90, volume=(32, 32, 24), voxel_mm=(0.8, 0.8, 0.8),
+                       channels=channels)
     _, proj = shepp_logan_projection(ctx, g, "joseph")
     plan32 = yk.FdkPlan.from_geometry(ctx, g)
     plan = yk.FdkPlan.from_geometry(ctx, g, storage=storage, kahan=kahan)
