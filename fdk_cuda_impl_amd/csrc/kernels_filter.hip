@@ -8,7 +8,7 @@
 // and both rows come out of one forward and one inverse transform.  The
 // transform is a mixed radix-8/4/2 Stockham FFT over two padded LDS ping-pong
 // buffers; twiddles come from an FP64-computed table, ordered pass by pass
-// (fft_twiddles), staged into LDS.  Equivalent to the reference's
+// (fft_twiddles), read from global memory.  Equivalent to the reference's
 // pad -> cuFFT R2C -> pointwise * w -> C2R -> crop chain
 // (src/Reconstruction/Analytic/Circular/Flat/FDK/kernels/YkFDKFilterKernels.cu:32-140,
 // src/Filter/YkConvKernels.cu:109-220) with the preweight of
@@ -201,9 +201,8 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
     const int padded = pidx(n);
     float2* bufa = reinterpret_cast<float2*>(smem_raw);
     float2* bufb = bufa + padded;
-    float2* tws = bufb + padded;
+    const float2* tws = tw;  // pass-ordered table, read through the L1
     __shared__ float red[kFilterThreads / 64];
-    for (int m = threadIdx.x; m < n; m += kFilterThreads) tws[m] = tw[m];
 
     const int pairs = (rows + 1) / 2;
     float xa[NPT], xb[NPT];
@@ -344,8 +343,9 @@ void launch_preweight(const FilterLaunch& p, float* out, hipStream_t s)
 void launch_fdk_filter(const FilterLaunch& p, hipStream_t s)
 {
     const int rows = p.count * p.nv;
-    // two padded ping-pong buffers and the twiddle table
-    const size_t lds = (size_t(2) * (p.n + p.n / 8) + size_t(p.n)) * sizeof(float2);
+    // two padded ping-pong buffers (the twiddles stay in global memory: an
+    // LDS copy would cost a workgroup per CU)
+    const size_t lds = (size_t(2) * (p.n + p.n / 8)) * sizeof(float2);
     if (lds > 160 * 1024) fail(YK_ERR_UNSUPPORTED, "padded row too long for the LDS filter");
     switch (p.storage) {
     case YK_F32: launch_storage<YK_F32>(p, rows, lds, s); break;
