@@ -82,3 +82,17 @@ def test_invalid_descriptors_are_refused(problem):
                dict(check_interval=0), dict(patience=0), dict(min_value=1.0, max_value=0.0)):
         with pytest.raises(yk.YkError):
             P.cgls_ex(b, **kw)
+
+
+def test_tilted_detector_robust_cgls(ctx, oracle):
+    g = small_geometry(total_views=12, views_per_turn=12, volume=(20, 18, 14), voxel_mm=(0.9, 1.1, 1.3),
+                       tilt_rad=(0.02, 0.01, -0.03))
+    P = yk.Projector(ctx, g.views(), g.nu, g.nv, g.grid())
+    truth = ctx.phantom(phantoms.shepp_logan(8.0, scale=1.0), g.grid())
+    b = P.forward(truth, kind="siddon")
+    x, res, st = P.cgls_ex(b, iterations=6, forward="siddon", back="siddon")
+    ox, ores, ost = oracle.cgls(oracle.views_from_float(P.views), g.nu, g.nv, oracle_volume(oracle, g),
+                                b.cpu().numpy(), iterations=6, forward="siddon", back="siddon")
+    assert st["completed_iterations"] == ost["completed_iterations"] == 6
+    assert rel_rmse(x.cpu().numpy(), ox) < TOL, rel_rmse(x.cpu().numpy(), ox)
+
