@@ -93,7 +93,12 @@ class AlgebraicDesc(ctypes.Structure):
                 ("use_max", ctypes.c_int), ("max_value", ctypes.c_float),
                 ("forward_kind", ctypes.c_int), ("back_kind", ctypes.c_int),
                 ("tv_strength", ctypes.c_float), ("tv_iterations", ctypes.c_int),
-                ("tv_epsilon", ctypes.c_float), ("tv_strength_reduction", ctypes.c_float)]
+                ("tv_epsilon", ctypes.c_float), ("tv_strength_reduction", ctypes.c_float),
+                ("relative_residual_tolerance", ctypes.c_float),
+                ("relative_update_tolerance", ctypes.c_float),
+                ("relative_improvement_tolerance", ctypes.c_float),
+                ("minimum_iterations", ctypes.c_int), ("check_interval", ctypes.c_int),
+                ("patience", ctypes.c_int)]
 
 
 class FdkStage(ctypes.Structure):
@@ -130,7 +135,7 @@ class CglsDesc(ctypes.Structure):
                 ("patience", ctypes.c_int)]
 
 
-class CglsStats(ctypes.Structure):
+class IterativeStats(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("completed_iterations", ctypes.c_int),
                 ("convergence_checks", ctypes.c_int), ("stopped_by_breakdown", ctypes.c_int),
                 ("stopped_by_divergence", ctypes.c_int), ("stopped_by_relative_residual", ctypes.c_int),
@@ -185,7 +190,7 @@ SIGNATURES = {
     "yk_project": (_I, [_P, _I, _P, _P, _I, _I, _I]),
     "yk_backproject": (_I, [_P, _I, _P, _P, _I, _I, _I]),
     "yk_cgls": (_I, [_P, _I, _I, _P, _P, _I, _DP]),
-    "yk_cgls_ex": (_I, [_P, ctypes.POINTER(CglsDesc), _P, _P, ctypes.POINTER(CglsStats), _DP]),
+    "yk_cgls_ex": (_I, [_P, ctypes.POINTER(CglsDesc), _P, _P, ctypes.POINTER(IterativeStats), _DP]),
     "yk_fdk_set_stage_capture": (_I, [_P, _I]),
     "yk_fdk_last_stage": (_I, [_P, ctypes.POINTER(FdkStage)]),
     "yk_fdk_geometry_diagnostics": (_I, [_P, ctypes.POINTER(GeometryDiagnostics)]),
@@ -198,7 +203,7 @@ SIGNATURES = {
     "yk_dump_text": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p]),
     "yk_dump_text_a": (_I, [_P, ctypes.c_char_p, _I, ctypes.c_char_p]),
     "yk_sart": (_I, [_P, _I, _I, _P, _P, _I, _F, _I]),
-    "yk_algebraic": (_I, [_P, ctypes.POINTER(AlgebraicDesc), _P, _P]),
+    "yk_algebraic": (_I, [_P, ctypes.POINTER(AlgebraicDesc), _P, _P, ctypes.POINTER(IterativeStats)]),
     "yk_tv_gradient": (_I, [_P, ctypes.POINTER(VolumeGrid), _P, _F, _P, _DP]),
     "yk_dot": (_I, [_P, _P, _P, _SZ, _DP]),
     "yk_phantom_ellipsoids": (_I, [_P, _FP, _I, ctypes.POINTER(VolumeGrid), _I, _P]),

@@ -528,12 +528,12 @@ class Projector:
         d.minimum_iterations = minimum_iterations
         d.check_interval = check_interval
         d.patience = patience
-        st = C.CglsStats()
-        st.struct_size = ctypes.sizeof(C.CglsStats)
+        st = C.IterativeStats()
+        st.struct_size = ctypes.sizeof(C.IterativeStats)
         res = np.zeros(max(iterations, 0) + 1, dtype=np.float64)
         C.call("yk_cgls_ex", self.handle, ctypes.byref(d), _ptr(b), _ptr(x), ctypes.byref(st),
                res.ctypes.data_as(C._DP))
-        stats = {name: getattr(st, name) for name, _ in C.CglsStats._fields_ if name != "struct_size"}
+        stats = {name: getattr(st, name) for name, _ in C.IterativeStats._fields_ if name != "struct_size"}
         return x, res, stats
 
     def sart(self, b, x=None, iterations=1, relaxation=1.0, nonnegative=False, forward="joseph",
@@ -549,7 +549,9 @@ class Projector:
     def algebraic(self, b, method="sart", x=None, iterations=10, relaxation=1.0, subsets=1,
                   weights=None, order=None, relaxation_reduction=1.0, epsilon=1e-6, min_value=None,
                   max_value=None, forward="joseph", back="joseph_voxel", tv_strength=0.0, tv_iterations=0,
-                  tv_epsilon=1e-8, tv_strength_reduction=1.0):
+                  tv_epsilon=1e-8, tv_strength_reduction=1.0, relative_residual=0.0, relative_update=0.0,
+                  relative_improvement=0.0, minimum_iterations=1, check_interval=1, patience=1,
+                  return_stats=False):
         """SIRT / SART / OS-SART on the device, updating x (zeros when None) in place.
 
         ``weights`` is "detailed" (exact per-subset row and column sums) or
@@ -592,7 +594,17 @@ class Projector:
         d.tv_iterations = tv_iterations
         d.tv_epsilon = tv_epsilon
         d.tv_strength_reduction = tv_strength_reduction
-        C.call("yk_algebraic", self.handle, ctypes.byref(d), _ptr(b), _ptr(x))
+        d.relative_residual_tolerance = relative_residual
+        d.relative_update_tolerance = relative_update
+        d.relative_improvement_tolerance = relative_improvement
+        d.minimum_iterations = minimum_iterations
+        d.check_interval = check_interval
+        d.patience = patience
+        st = C.IterativeStats()
+        st.struct_size = ctypes.sizeof(C.IterativeStats)
+        C.call("yk_algebraic", self.handle, ctypes.byref(d), _ptr(b), _ptr(x), ctypes.byref(st))
+        if return_stats:
+            return x, {name: getattr(st, name) for name, _ in C.IterativeStats._fields_ if name != "struct_size"}
         return x
 
     def close(self):

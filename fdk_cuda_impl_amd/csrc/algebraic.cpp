@@ -11,6 +11,7 @@
 // views, so every operator call is a plain [first, first + count) range.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <memory>
 #include <numeric>
 #include <vector>
@@ -59,7 +60,7 @@ std::vector<int> golden_order(int n)
 
 }  // namespace
 
-void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b, float* d_x)
+void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b, float* d_x, yk_iterative_stats* out)
 {
     const int na = p->n_views;
     if (d.method < YK_ALG_SIRT || d.method > YK_ALG_OSSART) fail(YK_ERR_INVALID_CONFIG, "unknown algebraic method");
@@ -82,6 +83,12 @@ void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b
         fail(YK_ERR_INVALID_CONFIG, "invalid TV strength, iterations, epsilon or reduction");
     if (d.forward_kind != YK_FP_JOSEPH && d.forward_kind != YK_FP_SIDDON)
         fail(YK_ERR_UNSUPPORTED, "unknown forward projector");
+    // validConvergenceConfig (src/Reconstruction/Iterative/Common/YkIterativeConvergence.hpp:39-46)
+    if (!(d.relative_residual_tolerance >= 0.f) || !(d.relative_update_tolerance >= 0.f) ||
+        !(d.relative_improvement_tolerance >= 0.f) || d.minimum_iterations < 0 || d.check_interval <= 0 ||
+        d.patience <= 0)
+        fail(YK_ERR_INVALID_CONFIG, "invalid convergence settings");
+    if (out && out->struct_size < sizeof(yk_iterative_stats)) fail(YK_ERR_INVALID_CONFIG, "yk_iterative_stats is truncated");
 
     p->ctx->activate();
     hipStream_t s = p->ctx->stream;
@@ -173,6 +180,20 @@ void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b
     }
     if (d.tv_iterations > 0 && d.tv_strength > 0.f) grad.resize(nx);
 
+    // convergence monitor state (AlgebraicReconstructorEx::prepareConvergence_, :236-256)
+    const bool watch_residual = d.relative_residual_tolerance > 0.f || d.relative_improvement_tolerance > 0.f;
+    const bool watch_update = d.relative_update_tolerance > 0.f;
+    DeviceArray<float> resid(watch_residual ? size_t(na) * view_n : 0), prev(watch_update ? nx : 0);
+    if (watch_update) YK_HIP(hipMemcpyAsync(prev.get(), d_x, nx * sizeof(float), hipMemcpyDeviceToDevice, s));
+    const double measured = watch_residual ? std::sqrt(std::max(0.0, p->dot(d_b, d_b, size_t(na) * view_n))) : 0.0;
+    const double eps = d.epsilon;
+    yk_iterative_stats st{};
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    st.projection_residual_l2 = st.relative_projection_residual = nan;
+    st.relative_volume_update = st.relative_residual_improvement = nan;
+    double previous_residual = nan;
+    int patience = 0;
+
     float lambda = d.relaxation;
     const float per_update = tigre ? 1.f : std::pow(d.relaxation_reduction, 1.f / float(nu_units));
     float tv_strength = d.tv_strength;
@@ -208,8 +229,49 @@ void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b
             tv_strength *= d.tv_strength_reduction;
         }
         if (d.use_min || d.use_max) launch_bounds_clamp(d_x, d.use_min, d.min_value, d.use_max, d.max_value, nx, s);
+        st.completed_iterations = outer + 1;
+        // shouldCheckConvergence_ / checkConvergence_ (YkAlgebraicReconstructorEx.hpp:258-330)
+        if ((!watch_residual && !watch_update) ||
+            ((outer + 1) % d.check_interval != 0 && outer + 1 != d.iterations))
+            continue;
+        ++st.convergence_checks;
+        bool residual_ok = false, update_ok = false, stagnation_ok = false;
+        if (watch_residual) {
+            p->project(d.forward_kind, d_x, resid.get(), 0, na, 0);
+            launch_xpby(resid.get(), d_b, -1.f, size_t(na) * view_n, s);  // b - A x
+            const double r = std::sqrt(std::max(0.0, p->dot(resid.get(), resid.get(), size_t(na) * view_n)));
+            st.projection_residual_l2 = r;
+            st.relative_projection_residual = r / std::max(measured, eps);
+            residual_ok = d.relative_residual_tolerance > 0.f &&
+                          st.relative_projection_residual <= d.relative_residual_tolerance;
+            if (std::isfinite(previous_residual)) {
+                st.relative_residual_improvement = (previous_residual - r) / std::max(previous_residual, eps);
+                stagnation_ok = d.relative_improvement_tolerance > 0.f && st.relative_residual_improvement >= 0.0 &&
+                                st.relative_residual_improvement <= d.relative_improvement_tolerance;
+            }
+            previous_residual = r;
+        }
+        if (watch_update) {
+            const double pn = std::sqrt(std::max(0.0, p->dot(prev.get(), prev.get(), nx)));
+            launch_xpby(prev.get(), d_x, -1.f, nx, s);  // x - x_prev
+            st.relative_volume_update = std::sqrt(std::max(0.0, p->dot(prev.get(), prev.get(), nx))) / std::max(pn, eps);
+            update_ok = st.relative_volume_update <= d.relative_update_tolerance;
+            YK_HIP(hipMemcpyAsync(prev.get(), d_x, nx * sizeof(float), hipMemcpyDeviceToDevice, s));
+        }
+        const bool satisfied =
+            st.completed_iterations >= d.minimum_iterations && (residual_ok || update_ok || stagnation_ok);
+        patience = satisfied ? patience + 1 : 0;
+        if (patience < d.patience) continue;
+        st.stopped_by_relative_residual = residual_ok;
+        st.stopped_by_relative_update = update_ok;
+        st.stopped_by_stagnation = stagnation_ok;
+        break;
     }
     // the sub-projectors and workspaces are released when this returns: wait
     // for the queued work first
     YK_HIP(hipStreamSynchronize(s));
+    if (out) {
+        st.struct_size = sizeof(yk_iterative_stats);
+        *out = st;
+    }
 }

@@ -547,7 +547,7 @@ yk_status yk_cgls(yk_projector* p, int fk, int bk, const float* d_b, float* d_x,
     });
 }
 
-yk_status yk_cgls_ex(yk_projector* p, const yk_cgls_desc* desc, const float* d_b, float* d_x, yk_cgls_stats* stats,
+yk_status yk_cgls_ex(yk_projector* p, const yk_cgls_desc* desc, const float* d_b, float* d_x, yk_iterative_stats* stats,
     double* residual_norms)
 {
     return guard([&] {
@@ -585,19 +585,22 @@ yk_status yk_sart(yk_projector* p, int fk, int bk, const float* d_b, float* d_x,
         d.forward_kind = fk;
         d.back_kind = bk;
         d.tv_strength_reduction = 1.f;
-        run_algebraic(p, d, d_b, d_x);
+        d.check_interval = 1;
+        d.patience = 1;
+        run_algebraic(p, d, d_b, d_x, nullptr);
         p->ctx->finish();
     });
 }
 
-yk_status yk_algebraic(yk_projector* p, const yk_algebraic_desc* desc, const float* d_b, float* d_x)
+yk_status yk_algebraic(yk_projector* p, const yk_algebraic_desc* desc, const float* d_b, float* d_x,
+    yk_iterative_stats* stats)
 {
     return guard([&] {
         require(p != nullptr && desc != nullptr, YK_ERR_INVALID_CONFIG, "null projector or descriptor");
         require(desc->struct_size >= sizeof(yk_algebraic_desc), YK_ERR_INVALID_CONFIG,
             "yk_algebraic_desc is truncated or from an older layout");
         require(d_b && d_x, YK_ERR_INVALID_BUFFER, "null device buffer");
-        run_algebraic(p, *desc, d_b, d_x);
+        run_algebraic(p, *desc, d_b, d_x, stats);
         p->ctx->finish();
     });
 }

@@ -27,7 +27,7 @@ bool convergence_enabled(const yk_cgls_desc& d)
 
 }  // namespace
 
-void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d_x, yk_cgls_stats* out,
+void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d_x, yk_iterative_stats* out,
     double* residual_norms)
 {
     // CglsReconstructorEx::prepare validation (YkCglsReconstructorEx.hpp:45-51)
@@ -45,7 +45,7 @@ void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d
         fail(YK_ERR_INVALID_CONFIG, "the ASTRA strategy has no convergence monitor");
     if (d.forward_kind != YK_FP_JOSEPH && d.forward_kind != YK_FP_SIDDON)
         fail(YK_ERR_UNSUPPORTED, "unknown forward projector");
-    if (out && out->struct_size < sizeof(yk_cgls_stats)) fail(YK_ERR_INVALID_CONFIG, "yk_cgls_stats is truncated");
+    if (out && out->struct_size < sizeof(yk_iterative_stats)) fail(YK_ERR_INVALID_CONFIG, "yk_iterative_stats is truncated");
 
     p->ctx->activate();
     hipStream_t s = p->ctx->stream;
@@ -53,7 +53,7 @@ void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d
     const size_t np = size_t(n) * p->nu * p->nv;
     const size_t nx = size_t(p->grid.nx) * p->grid.ny * p->grid.nz;
     const double eps = d.epsilon;
-    yk_cgls_stats st{};
+    yk_iterative_stats st{};
     st.projection_residual_l2 = std::numeric_limits<double>::quiet_NaN();
     st.relative_projection_residual = std::numeric_limits<double>::quiet_NaN();
     st.relative_volume_update = std::numeric_limits<double>::quiet_NaN();
@@ -189,7 +189,7 @@ void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d
     }
     YK_HIP(hipStreamSynchronize(s));
     if (out) {
-        st.struct_size = sizeof(yk_cgls_stats);
+        st.struct_size = sizeof(yk_iterative_stats);
         *out = st;
     }
 }

@@ -103,9 +103,10 @@ struct yk_projector {
 };
 
 // unified SIRT / SART / OS-SART (algebraic.cpp); throws through fail()
-void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b, float* d_x);
+void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b, float* d_x,
+    yk_iterative_stats* stats);
 // CGLS, robust-restart and ASTRA strategies (cgls.cpp); throws through fail()
-void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d_x, yk_cgls_stats* stats,
+void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d_x, yk_iterative_stats* stats,
     double* residual_norms);
 
 // calibrated-geometry diagnostics (diagnostics.cpp); throws INVALID_GEOMETRY

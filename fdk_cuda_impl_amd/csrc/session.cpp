@@ -400,7 +400,7 @@ private:
             d.back_kind = back_kind_;
             d.check_interval = 1;
             d.patience = 1;
-            yk_cgls_stats st{};
+            yk_iterative_stats st{};
             st.struct_size = sizeof(st);
             if (yk_status e = yk_cgls_ex(proj_, &d, d_b, r.volume.data, &st, nullptr); e != YK_OK)
                 return fail(EApiErrorCode::CudaError, std::string("CGLS: ") + yk_last_error());
@@ -421,7 +421,9 @@ private:
             d.forward_kind = forward_kind_;
             d.back_kind = back_kind_;
             d.tv_strength_reduction = 1.f;
-            if (yk_status e = yk_algebraic(proj_, &d, d_b, r.volume.data); e != YK_OK)
+            d.check_interval = 1;
+            d.patience = 1;
+            if (yk_status e = yk_algebraic(proj_, &d, d_b, r.volume.data, nullptr); e != YK_OK)
                 return fail(EApiErrorCode::CudaError, std::string("algebraic reconstruction: ") + yk_last_error());
             done = iterations;
         }

@@ -378,8 +378,10 @@ typedef struct yk_cgls_desc {
     int check_interval;              /* >= 1 */
     int patience;                    /* >= 1 checks in a row */
 } yk_cgls_desc;
-typedef struct yk_cgls_stats {
-    uint32_t struct_size;            /* caller sets sizeof(yk_cgls_stats) */
+/* IterativeConvergenceStatistics (src/Reconstruction/Iterative/Common/
+ * YkIterativeConvergence.hpp:18-30) plus the CGLS stop reasons */
+typedef struct yk_iterative_stats {
+    uint32_t struct_size;            /* caller sets sizeof(yk_iterative_stats) */
     int completed_iterations;
     int convergence_checks;
     int stopped_by_breakdown;
@@ -391,9 +393,9 @@ typedef struct yk_cgls_stats {
     double relative_projection_residual;
     double relative_volume_update;
     double relative_residual_improvement;
-} yk_cgls_stats;
+} yk_iterative_stats;
 yk_status yk_cgls_ex(yk_projector* projector, const yk_cgls_desc* desc, const float* d_b, float* d_x,
-    yk_cgls_stats* stats, double* residual_norms);
+    yk_iterative_stats* stats, double* residual_norms);
 /* SART with view-by-view updates (AlgebraicReconstructorEx,
  * src/Reconstruction/Iterative/Flat/YkAlgebraicReconstructorEx.hpp:137-180) */
 yk_status yk_sart(yk_projector* projector, int forward_kind, int back_kind,
@@ -440,9 +442,23 @@ typedef struct yk_algebraic_desc {
     int tv_iterations;           /* TV steps after each outer iteration */
     float tv_epsilon;
     float tv_strength_reduction; /* strength *= this after each outer iteration */
+    /* convergence monitor after each checked outer iteration (all tolerances
+     * 0: off; AlgebraicReconstructorEx::checkConvergence_,
+     * YkAlgebraicReconstructorEx.hpp:258-330): ||b - A x|| / ||b||, the
+     * residual's relative improvement since the last check, ||x - x_prev|| /
+     * ||x_prev|| since the last check; stop after `patience` satisfied checks
+     * in a row at or past minimum_iterations */
+    float relative_residual_tolerance;
+    float relative_update_tolerance;
+    float relative_improvement_tolerance;
+    int minimum_iterations;          /* >= 0 */
+    int check_interval;              /* >= 1 */
+    int patience;                    /* >= 1 */
 } yk_algebraic_desc;
+/* stats (optional) receives the completed outer iterations and the monitor's
+ * last readings */
 yk_status yk_algebraic(yk_projector* projector, const yk_algebraic_desc* desc,
-    const float* d_b, float* d_x);
+    const float* d_b, float* d_x, yk_iterative_stats* stats);
 /* smoothed TV value and gradient (YkTvRegularizationKernels.cu:17-73) */
 yk_status yk_tv_gradient(yk_ctx* ctx, const yk_volume_grid* volume, const float* d_x,
     float epsilon, float* d_gradient, double* value);

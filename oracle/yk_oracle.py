@@ -393,9 +393,12 @@ def tigre_row_weights(views, nu, nv, vol: Volume, forward="joseph") -> np.ndarra
 def algebraic(views, nu, nv, vol: Volume, b, method="sart", weights=None, order=None, iterations=10,
               subsets=1, relaxation=1.0, relaxation_reduction=1.0, epsilon=1e-6, min_value=None,
               max_value=None, forward="joseph", back="joseph_voxel", tv_strength=0.0, tv_iterations=0,
-              tv_epsilon=1e-8, tv_strength_reduction=1.0, x0=None):
+              tv_epsilon=1e-8, tv_strength_reduction=1.0, x0=None, relative_residual=0.0,
+              relative_update=0.0, relative_improvement=0.0, minimum_iterations=1, check_interval=1,
+              patience=1, return_stats=False):
     """FP64 restatement of the unified algebraic reconstructor over the
-    oracle's projector pair; returns the volume [z][y][x].
+    oracle's projector pair; returns the volume [z][y][x] (and the monitor's
+    statistics with ``return_stats``).
 
     Outer loop (AlgebraicReconstructorEx::reconstruct,
     src/Reconstruction/Iterative/Flat/YkAlgebraicReconstructorEx.hpp:137-180):
@@ -419,6 +422,14 @@ def algebraic(views, nu, nv, vol: Volume, b, method="sart", weights=None, order=
     volume, with A_lo 1 <= min voxel / 2 -> 1e30 before the reciprocal;
     C = B_s 1 with C <= 0 -> 1e30; r = (b_s - A_s x) W; x += lambda B_s r /
     (C + eps), bounds; lambda *= reduction once per sweep.
+
+    Convergence monitor (AlgebraicReconstructorEx::reconstruct / checkConvergence_,
+    YkAlgebraicReconstructorEx.hpp:150-170, 258-330): after outer iteration k
+    (every ``check_interval``-th and the last), r = |b - A x| / max(|b|, eps),
+    improvement (r_prev - r) / max(r_prev, eps) since the last check, update
+    |x - x_prev| / max(|x_prev|, eps) with x_prev the volume at the last check
+    (or the start); satisfied when k >= minimum_iterations and any enabled
+    test holds; stop after ``patience`` satisfied checks in a row.
     """
     views = _d(views)
     na = views.shape[0]
@@ -465,7 +476,16 @@ def algebraic(views, nu, nv, vol: Volume, b, method="sart", weights=None, order=
     lam = float(relaxation)
     per_update = 1.0 if tigre else float(relaxation_reduction) ** (1.0 / len(units))
     strength = float(tv_strength)
-    for _ in range(iterations):
+    watch_res = relative_residual > 0 or relative_improvement > 0
+    watch_upd = relative_update > 0
+    measured = float(np.linalg.norm(b)) if watch_res else 0.0
+    x_prev = x.copy()
+    st = dict(completed_iterations=0, convergence_checks=0, stopped_by_relative_residual=0,
+              stopped_by_relative_update=0, stopped_by_stagnation=0, projection_residual_l2=np.nan,
+              relative_projection_residual=np.nan, relative_volume_update=np.nan,
+              relative_residual_improvement=np.nan)
+    prev_res, count = np.nan, 0
+    for outer in range(iterations):
         for k in visit:
             u = units[k]
             res = b[u] - A(views[u], x)
@@ -483,7 +503,34 @@ def algebraic(views, nu, nv, vol: Volume, b, method="sart", weights=None, order=
                 x = x - strength * g
             strength *= tv_strength_reduction
         x = bounds(x)
-    return x
+        done = outer + 1
+        st["completed_iterations"] = done
+        if not (watch_res or watch_upd) or (done % check_interval != 0 and done != iterations):
+            continue
+        st["convergence_checks"] += 1
+        res_ok = upd_ok = stag_ok = False
+        if watch_res:
+            r = float(np.linalg.norm(b - A(views, x)))
+            st["projection_residual_l2"] = r
+            st["relative_projection_residual"] = r / max(measured, epsilon)
+            res_ok = relative_residual > 0 and st["relative_projection_residual"] <= relative_residual
+            if np.isfinite(prev_res):
+                st["relative_residual_improvement"] = (prev_res - r) / max(prev_res, epsilon)
+                stag_ok = (relative_improvement > 0 and st["relative_residual_improvement"] >= 0
+                           and st["relative_residual_improvement"] <= relative_improvement)
+            prev_res = r
+        if watch_upd:
+            st["relative_volume_update"] = float(np.linalg.norm(x - x_prev) / max(np.linalg.norm(x_prev), epsilon))
+            upd_ok = st["relative_volume_update"] <= relative_update
+            x_prev = x.copy()
+        ok = done >= minimum_iterations and (res_ok or upd_ok or stag_ok)
+        count = count + 1 if ok else 0
+        if count >= patience:
+            st["stopped_by_relative_residual"] = int(res_ok)
+            st["stopped_by_relative_update"] = int(upd_ok)
+            st["stopped_by_stagnation"] = int(stag_ok)
+            break
+    return (x, st) if return_stats else x
 
 
 # --- CGLS (robust-restart and ASTRA strategies) ------------------------------

@@ -644,7 +644,9 @@ Result iterative_mapping()
             d.forward_kind = YK_FP_SIDDON;
             d.back_kind = c.bk;
             d.tv_strength_reduction = 1.f;
-            st = yk_algebraic(pr, &d, proj.p, b.p);
+            d.check_interval = 1;
+            d.patience = 1;
+            st = yk_algebraic(pr, &d, proj.p, b.p, nullptr);
         }
         if (st != YK_OK) return { false, yk_last_error() };
         const auto va = a.download(), vb = b.download();

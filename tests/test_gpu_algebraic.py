@@ -129,3 +129,28 @@ def test_tilted_detector_takes_the_general_operators(ctx, oracle, kw):
                            b.cpu().numpy(), iterations=2, **kw)
     assert rel_rmse(got, ref) < TOL, rel_rmse(got, ref)
 
+
+
+def test_reference_convergence_known_answer(problem):
+    """Yktest/test_reconstructors_smoke.cpp:184-205: relative residual
+    tolerance 2 (> 1) stops OS-SART at the first check, after one iteration."""
+    g, P, b, ov, vol = problem
+    kw = dict(method="ossart", subsets=2, relaxation=0.2, min_value=0.0)
+    x, st = P.algebraic(b, iterations=4, relative_residual=2.0, return_stats=True, **kw)
+    assert st["completed_iterations"] == 1 and st["convergence_checks"] == 1
+    assert st["stopped_by_relative_residual"] == 1 and np.isfinite(st["relative_projection_residual"])
+    one = P.algebraic(b, iterations=1, **kw)
+    np.testing.assert_array_equal(x.cpu().numpy(), one.cpu().numpy())
+
+
+def test_update_tolerance_matches_oracle(oracle, problem):
+    g, P, b, ov, vol = problem
+    kw = dict(method="sirt", relaxation=1.0, iterations=20, relative_update=0.05, minimum_iterations=2,
+              check_interval=2, patience=1, back="joseph")
+    x, st = P.algebraic(b, return_stats=True, **kw)
+    ox, ost = oracle.algebraic(ov, g.nu, g.nv, vol, b.cpu().numpy(), return_stats=True, **kw)
+    assert ost["stopped_by_relative_update"] == 1 and ost["completed_iterations"] < 20
+    assert st["completed_iterations"] == ost["completed_iterations"]
+    assert st["convergence_checks"] == ost["convergence_checks"]
+    assert st["relative_volume_update"] == pytest.approx(ost["relative_volume_update"], rel=1e-3)
+    assert rel_rmse(x.cpu().numpy(), ox) < TOL

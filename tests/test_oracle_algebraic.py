@@ -209,3 +209,46 @@ def test_matches_explicit_matrix_restatement(oracle, tiny, weights, method, subs
                          tv_strength=tv[0], tv_iterations=tv[1], tv_epsilon=tv[2],
                          tv_strength_reduction=tv[3])
     np.testing.assert_allclose(x, expect, rtol=1e-6, atol=1e-7)
+
+
+def test_convergence_known_answer(oracle, tiny):
+    """Yktest/test_reconstructors_smoke.cpp:184-205: OS-SART, 4 iterations, 2
+    subsets, relaxation 0.2, a relative-residual tolerance of 2 (> 1) stops at
+    the first check after one iteration."""
+    views, vol = tiny
+    A = system_matrix(oracle, views, vol)
+    b = A @ np.random.default_rng(4).random(A.shape[1])
+    kw = dict(method="ossart", subsets=2, relaxation=0.2, min_value=0.0, back="joseph")
+    x, st = oracle.algebraic(views, NU, NV, vol, b, iterations=4, relative_residual=2.0, return_stats=True, **kw)
+    assert st["completed_iterations"] == 1 and st["convergence_checks"] == 1
+    assert st["stopped_by_relative_residual"] == 1
+    np.testing.assert_array_equal(x, oracle.algebraic(views, NU, NV, vol, b, iterations=1, **kw))
+    # the residual is |b - A x| / |b| on the explicit matrix
+    assert st["relative_projection_residual"] == pytest.approx(
+        np.linalg.norm(b - A @ x.ravel()) / np.linalg.norm(b), rel=1e-6)
+
+
+@pytest.mark.parametrize("interval,patience", [(1, 1), (2, 1), (1, 2)])
+def test_update_tolerance_stop_matches_recomputation(oracle, tiny, interval, patience):
+    views, vol = tiny
+    A = system_matrix(oracle, views, vol)
+    b = A @ np.random.default_rng(8).random(A.shape[1])
+    kw = dict(method="sirt", relaxation=1.0, back="joseph")
+    xs = [np.zeros(SHAPE)] + [oracle.algebraic(views, NU, NV, vol, b, iterations=k, **kw) for k in range(1, 13)]
+    tol = 0.2
+    # recompute the checks: every `interval`-th iteration and the last (12)
+    expect, prev, count = 12, xs[0], 0
+    for k in range(1, 13):
+        if k % interval and k != 12:
+            continue
+        ratio = np.linalg.norm(xs[k] - prev) / max(np.linalg.norm(prev), 1e-6)
+        prev = xs[k]
+        count = count + 1 if (k >= 2 and ratio <= tol) else 0
+        if count >= patience:
+            expect = k
+            break
+    assert expect < 12
+    x, st = oracle.algebraic(views, NU, NV, vol, b, iterations=12, relative_update=tol, minimum_iterations=2,
+                             check_interval=interval, patience=patience, return_stats=True, **kw)
+    assert st["completed_iterations"] == expect and st["stopped_by_relative_update"] == 1
+    np.testing.assert_allclose(x, xs[expect], rtol=1e-12, atol=1e-14)
