@@ -98,7 +98,7 @@ class AlgebraicDesc(ctypes.Structure):
                 ("relative_update_tolerance", ctypes.c_float),
                 ("relative_improvement_tolerance", ctypes.c_float),
                 ("minimum_iterations", ctypes.c_int), ("check_interval", ctypes.c_int),
-                ("patience", ctypes.c_int)]
+                ("patience", ctypes.c_int), ("tv_dimensions", ctypes.c_int)]
 
 
 class FdkStage(ctypes.Structure):

@@ -549,7 +549,8 @@ class Projector:
     def algebraic(self, b, method="sart", x=None, iterations=10, relaxation=1.0, subsets=1,
                   weights=None, order=None, relaxation_reduction=1.0, epsilon=1e-6, min_value=None,
                   max_value=None, forward="joseph", back="joseph_voxel", tv_strength=0.0, tv_iterations=0,
-                  tv_epsilon=1e-8, tv_strength_reduction=1.0, relative_residual=0.0, relative_update=0.0,
+                  tv_epsilon=1e-8, tv_strength_reduction=1.0, tv_dimensions=3, relative_residual=0.0,
+                  relative_update=0.0,
                   relative_improvement=0.0, minimum_iterations=1, check_interval=1, patience=1,
                   return_stats=False):
         """SIRT / SART / OS-SART on the device, updating x (zeros when None) in place.
@@ -594,6 +595,7 @@ class Projector:
         d.tv_iterations = tv_iterations
         d.tv_epsilon = tv_epsilon
         d.tv_strength_reduction = tv_strength_reduction
+        d.tv_dimensions = tv_dimensions
         d.relative_residual_tolerance = relative_residual
         d.relative_update_tolerance = relative_update
         d.relative_improvement_tolerance = relative_improvement

@@ -81,6 +81,8 @@ void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b
         (d.tv_iterations > 0 && d.tv_strength > 0.f && !(d.tv_epsilon > 0.f)) ||
         !std::isfinite(d.tv_strength_reduction) || !(d.tv_strength_reduction > 0.f))
         fail(YK_ERR_INVALID_CONFIG, "invalid TV strength, iterations, epsilon or reduction");
+    if (d.tv_dimensions != 0 && d.tv_dimensions != 2 && d.tv_dimensions != 3)
+        fail(YK_ERR_INVALID_CONFIG, "TV dimensionality must be 2 or 3");
     if (d.forward_kind != YK_FP_JOSEPH && d.forward_kind != YK_FP_SIDDON)
         fail(YK_ERR_UNSUPPORTED, "unknown forward projector");
     // validConvergenceConfig (src/Reconstruction/Iterative/Common/YkIterativeConvergence.hpp:39-46)
@@ -223,7 +225,8 @@ void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b
         if (d.tv_iterations > 0 && tv_strength > 0.f) {
             p->ctx->reduce_scratch.resize(1024);
             for (int j = 0; j < d.tv_iterations; ++j) {
-                launch_tv_gradient(d_x, grad.get(), p->ctx->reduce_scratch.get(), 1024, g, d.tv_epsilon, s);
+                launch_tv_gradient(d_x, grad.get(), p->ctx->reduce_scratch.get(), 1024, g, d.tv_epsilon, s,
+                    d.tv_dimensions == 2 ? 2 : 3);
                 launch_axpy(d_x, grad.get(), -tv_strength, nx, s);
             }
             tv_strength *= d.tv_strength_reduction;

@@ -128,12 +128,13 @@ __device__ __forceinline__ void tv_field(const float* __restrict__ x, int i, int
 
 // gradient = -div(grad x / sqrt(|grad x|^2 + eps^2)), value = sum sqrt(...)
 // (YkTvRegularizationKernels.cu:41-73; doc/algorithms.md:257-261)
+// (iz = 0: the slice-wise 2-D form, no z differences; TvDimensionality::Slice2D)
 __global__ void __launch_bounds__(kThreads)
 tv_kernel(const float* __restrict__ x, float* __restrict__ grad, double* __restrict__ partials,
-    VolumeGeom g, float eps)
+    VolumeGeom g, float eps, float iz)
 {
     __shared__ double red[kThreads / 64];
-    const float ix = 1.f / g.vx, iy = 1.f / g.vy, iz = 1.f / g.vz;
+    const float ix = 1.f / g.vx, iy = 1.f / g.vy;
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     double val = 0.0;
     for (size_t c = (size_t)blockIdx.x * kThreads + threadIdx.x; c < n; c += (size_t)gridDim.x * kThreads) {
@@ -249,9 +250,10 @@ void weight_prepare(float* x, float threshold, int reciprocal, size_t n, hipStre
 }
 
 void launch_tv_gradient(const float* x, float* grad, double* partials, int n_partials,
-    const VolumeGeom& g, float eps, hipStream_t s)
+    const VolumeGeom& g, float eps, hipStream_t s, int dims)
 {
-    hipLaunchKernelGGL(tv_kernel, dim3(n_partials), dim3(kThreads), 0, s, x, grad, partials, g, eps);
+    hipLaunchKernelGGL(tv_kernel, dim3(n_partials), dim3(kThreads), 0, s, x, grad, partials, g, eps,
+        dims == 2 ? 0.f : 1.f / g.vz);
     YK_HIP(hipGetLastError());
 }
 

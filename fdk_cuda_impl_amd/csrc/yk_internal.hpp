@@ -398,8 +398,9 @@ void algebraic_update(float* x, const float* bp, const float* col, float lambda,
 void launch_bounds_clamp(float* x, int use_min, float lo, int use_max, float hi, size_t n, hipStream_t s);
 // x <= threshold -> 1e30, then optionally x = 1 / x
 void weight_prepare(float* x, float threshold, int reciprocal, size_t n, hipStream_t s);
+// smoothed TV value partials and gradient; dims 3 (volume) or 2 (each z slice alone)
 void launch_tv_gradient(const float* x, float* grad, double* partials, int n_partials,
-    const VolumeGeom& g, float eps, hipStream_t s);
+    const VolumeGeom& g, float eps, hipStream_t s, int dims = 3);
 void launch_phantom(const float* ellipsoids, int n, const VolumeGeom& g, int clamp,
     float* out, hipStream_t s);
 

@@ -454,6 +454,7 @@ typedef struct yk_algebraic_desc {
     int minimum_iterations;          /* >= 0 */
     int check_interval;              /* >= 1 */
     int patience;                    /* >= 1 */
+    int tv_dimensions;               /* 3 (Volume3D; 0 means 3) or 2 (Slice2D: no z differences) */
 } yk_algebraic_desc;
 /* stats (optional) receives the completed outer iterations and the monitor's
  * last readings */

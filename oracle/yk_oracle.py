@@ -393,7 +393,7 @@ def tigre_row_weights(views, nu, nv, vol: Volume, forward="joseph") -> np.ndarra
 def algebraic(views, nu, nv, vol: Volume, b, method="sart", weights=None, order=None, iterations=10,
               subsets=1, relaxation=1.0, relaxation_reduction=1.0, epsilon=1e-6, min_value=None,
               max_value=None, forward="joseph", back="joseph_voxel", tv_strength=0.0, tv_iterations=0,
-              tv_epsilon=1e-8, tv_strength_reduction=1.0, x0=None, relative_residual=0.0,
+              tv_epsilon=1e-8, tv_strength_reduction=1.0, tv_dimensions=3, x0=None, relative_residual=0.0,
               relative_update=0.0, relative_improvement=0.0, minimum_iterations=1, check_interval=1,
               patience=1, return_stats=False):
     """FP64 restatement of the unified algebraic reconstructor over the
@@ -498,8 +498,11 @@ def algebraic(views, nu, nv, vol: Volume, b, method="sart", weights=None, order=
         if tigre:
             lam *= relaxation_reduction
         if tv_iterations > 0 and strength > 0:
+            # Slice2D: no z differences, i.e. an infinite z spacing
+            tv_vol = vol if tv_dimensions != 2 else volume(vol.nx, vol.ny, vol.nz, vol.vx, vol.vy, np.inf,
+                                                           (vol.cx, vol.cy, vol.cz))
             for _ in range(tv_iterations):
-                _, g = tv_value_grad(x, vol, tv_epsilon)
+                _, g = tv_value_grad(x, tv_vol, tv_epsilon)
                 x = x - strength * g
             strength *= tv_strength_reduction
         x = bounds(x)

@@ -154,3 +154,12 @@ def test_update_tolerance_matches_oracle(oracle, problem):
     assert st["convergence_checks"] == ost["convergence_checks"]
     assert st["relative_volume_update"] == pytest.approx(ost["relative_volume_update"], rel=1e-3)
     assert rel_rmse(x.cpu().numpy(), ox) < TOL
+
+
+def test_slice_wise_tv(oracle, problem):
+    got, ref = run_both(oracle, problem, method="ossart", subsets=2, iterations=2, relaxation=0.2,
+                        min_value=0.0, tv_strength=2e-3, tv_iterations=3, tv_epsilon=1e-3, tv_dimensions=2)
+    assert rel_rmse(got, ref) < TOL, rel_rmse(got, ref)
+    vol3, _ = run_both(oracle, problem, method="ossart", subsets=2, iterations=2, relaxation=0.2,
+                       min_value=0.0, tv_strength=2e-3, tv_iterations=3, tv_epsilon=1e-3)
+    assert rel_rmse(got, vol3) > 1e-4  # the z differences matter

@@ -252,3 +252,25 @@ def test_update_tolerance_stop_matches_recomputation(oracle, tiny, interval, pat
                              check_interval=interval, patience=patience, return_stats=True, **kw)
     assert st["completed_iterations"] == expect and st["stopped_by_relative_update"] == 1
     np.testing.assert_allclose(x, xs[expect], rtol=1e-12, atol=1e-14)
+
+
+def test_slice_tv_is_the_volume_tv_without_z(oracle):
+    """The oracle's slice-wise TV (an infinite z spacing) against a numpy
+    per-slice restatement: forward differences in x and y with zero past the
+    edge, p = g / sqrt(|g|^2 + eps^2), gradient = -div p (backward)."""
+    rng = np.random.default_rng(11)
+    x = rng.random((4, 6, 7))
+    vol = oracle.volume(7, 6, 4, 0.8, 1.3, np.inf)
+    eps = 1e-2
+    val, g = oracle.tv_value_grad(x, vol, eps)
+    gx = np.zeros_like(x)
+    gy = np.zeros_like(x)
+    gx[:, :, :-1] = (x[:, :, 1:] - x[:, :, :-1]) / 0.8
+    gy[:, :-1, :] = (x[:, 1:, :] - x[:, :-1, :]) / 1.3
+    den = np.sqrt(gx ** 2 + gy ** 2 + eps ** 2)
+    px, py = gx / den, gy / den
+    div = px / 0.8 + py / 1.3
+    div[:, :, 1:] -= px[:, :, :-1] / 0.8
+    div[:, 1:, :] -= py[:, :-1, :] / 1.3
+    np.testing.assert_allclose(g, -div, rtol=1e-12, atol=1e-12)
+    assert val == pytest.approx(den.sum(), rel=1e-12)
