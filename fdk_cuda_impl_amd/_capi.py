@@ -218,6 +218,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: run fdk_cuda_impl_amd.build.build() first")
+        # PyTorch ships its own HIP runtime: load it before this library so the
+        # process holds one runtime (a HIP runtime initialized here first
+        # leaves torch.cuda without devices)
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

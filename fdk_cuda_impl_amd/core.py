@@ -170,10 +170,9 @@ class Context:
     def __init__(self, device: int = 0, use_torch_stream: bool = True, synchronous: bool = True):
         self.device = device
         self.handle = ctypes.c_void_p()
+        s = _torch().cuda.current_stream(device) if use_torch_stream else None
         C.call("yk_ctx_create", device, ctypes.byref(self.handle))
-        if use_torch_stream:
-            torch = _torch()
-            s = torch.cuda.current_stream(device)
+        if s is not None:
             C.call("yk_ctx_set_stream", self.handle, ctypes.c_void_p(s.cuda_stream))
         C.call("yk_ctx_set_synchronous", self.handle, 1 if synchronous else 0)
 
