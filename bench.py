@@ -187,6 +187,22 @@ def main():
     print(f"[bench] fp32 backprojection {ms32:.2f} ms", file=sys.stderr, flush=True)
     ms16 = time_launches(lambda: plan16.backproject(f16, s16, volume=v16), args.steps, args.warmup, stream)
     print(f"[bench] fp16 backprojection {ms16:.2f} ms", file=sys.stderr, flush=True)
+    # the general (per-sample perspective) path: the same scan with a detector
+    # tilted by a few mrad, as a calibrated scanner reports it
+    tgeom = yk.ScanGeometry(args.views, args.views, 600.0, 1100.0, 1024, 768, 0.3, 0.3,
+                            volume=(s, s, s), voxel_mm=(vox, vox, vox), offset_unv_mm=(0.75, 0.0, 0.0),
+                            tilt_rad=(0.002, -0.001, 0.003))
+    general = {"workload": "as above, detector tilted (0.002, -0.001, 0.003) rad about U, V, N"}
+    vt = torch.empty_like(v16)
+    for storage, kahan in (("f32", False), ("f16", True)):
+        tplan = yk.FdkPlan.from_geometry(ctx, tgeom, filter="hann", storage=storage, kahan=kahan)
+        tf, ts = tplan.filter(proj)
+        general[f"{storage}{'_kahan' if kahan else ''}_ms"] = round(
+            time_launches(lambda: tplan.backproject(tf, ts, volume=vt), 3, 1, stream), 3)
+        tplan.close()
+        del tf, ts
+    del vt
+    print(f"[bench] general path {general}", file=sys.stderr, flush=True)
     proj_leg = None
     if not args.no_projectors:
         proj_leg = projector_leg(yk, phantoms, ctx, stream)
@@ -219,6 +235,7 @@ def main():
         "filter_ms_fp16": round(filt_ms, 3),
         "joseph_fp_s": round(fp_s, 3),
         "projectors": proj_leg,
+        "general_path_backprojection": general,
     }
     if not args.no_cpu:
         print("[bench] FP64 oracle check", file=sys.stderr, flush=True)
