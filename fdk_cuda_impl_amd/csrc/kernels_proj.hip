@@ -190,11 +190,19 @@ joseph_fp_kernel(const float* __restrict__ vol, const float* __restrict__ vol_t,
 
 // Joseph projection for separable views (main axis X or Y, detV along Z
 // only).  One wave per detector column (64 consecutive rays in v): a1, b1 and
-// hence f1 and its two first-minor taps are the same for every lane, and the
-// lanes' Z taps are neighbouring entries of a Z-fastest volume copy
-// ([x][y][z] for X-main, [y][x][z] for Y-main views), so the four gathers of
-// a sample touch a few cache lines per wave.  Sampling rule, slice range,
-// weights and summation order are those of joseph_fp_kernel (same results).
+// hence f1 and its taps are the same for every lane, and the lanes' Z taps are
+// neighbouring entries of a Z-fastest volume copy ([x][y][z] for X-main,
+// [y][x][z] for Y-main views), so the gathers of a sample touch a few cache
+// lines per wave.  The copy carries kJosephGuard zero planes on both sides of
+// both minor axes: with the sample position clamped to [-2, n] both taps of
+// every sample lie in the copy, and a position outside (-1, n) reads only
+// zeros, so the kernel needs no per-sample range test (a slice outside a
+// lane's own range is outside (-1, n) on some minor axis).  A sample is two
+// 8-byte loads, (j1, j2..j2+1) and (j1 + 1, j2..j2+1), and three lerps.
+// Sample positions are those of joseph_fp_kernel (the same IEEE operations).
+constexpr int kJosephGuard = 2;
+typedef float float2u __attribute__((ext_vector_type(2), aligned(4)));
+
 __global__ void __launch_bounds__(256)
 joseph_fp_sep_kernel(const float* __restrict__ vzx, const float* __restrict__ vzy,
     float* __restrict__ proj, const ProjViewDev* __restrict__ views, VolumeGeom g,
@@ -216,37 +224,42 @@ joseph_fp_sep_kernel(const float* __restrict__ vzx, const float* __restrict__ vz
         wlo = min(wlo, __shfl_xor(wlo, off, 64));
         whi = max(whi, __shfl_xor(whi, off, 64));
     }
-    const int n1 = ax.n1, n2 = ax.n2;
+    wlo = __builtin_amdgcn_readfirstlane(wlo);
+    whi = __builtin_amdgcn_readfirstlane(whi);
+    const int p2 = ax.n2 + 2 * kJosephGuard;
+    const unsigned p2b = unsigned(p2) * 4u;
+    const size_t slice = size_t(ax.n1 + 2 * kJosephGuard) * p2;
+    const float f1max = float(ax.n1), f2max = float(ax.n2);
     float acc = 0.f;
     constexpr int kUnroll = 4;
     for (int s0 = wlo; s0 <= whi; s0 += kUnroll) {
-        float t[kUnroll][4], wa[kUnroll], wb[kUnroll], wc[kUnroll], wd[kUnroll];
+        float2u ta[kUnroll], tb[kUnroll];
+        float w1[kUnroll], w2[kUnroll];
 #pragma unroll
         for (int k = 0; k < kUnroll; ++k) {
             const int s = min(s0 + k, whi);
             float f1, f2;
             joseph_sample(r, ax, s, f1, f2);
-            const float g1 = floorf(f1), g2 = floorf(f2);
-            const float w1 = f1 - g1, w2 = f2 - g2;
-            const bool live = s0 + k <= whi && s >= lo && s <= hi && g1 >= -1.f && g2 >= -1.f &&
-                g1 <= float(n1 - 1) && g2 <= float(n2 - 1);
-            const int j1 = live ? int(g1) : 0, j2 = live ? int(g2) : 0;
-            const bool a0 = j1 >= 0, a1 = j1 + 1 < n1, b0 = j2 >= 0, b1 = j2 + 1 < n2;
-            const float x0 = live && a0 ? 1.f - w1 : 0.f, x1 = live && a1 ? w1 : 0.f;
-            const float y0 = b0 ? 1.f - w2 : 0.f, y1 = b1 ? w2 : 0.f;
-            wa[k] = x0 * y0; wb[k] = x1 * y0; wc[k] = x0 * y1; wd[k] = x1 * y1;
-            const int c1 = max(j1, 0), c1b = min(j1 + 1, n1 - 1);
-            const int c2 = max(j2, 0), c2b = min(j2 + 1, n2 - 1);
-            const float* r0 = base + ((size_t)s * n1 + c1) * n2;
-            const float* r1 = base + ((size_t)s * n1 + c1b) * n2;
-            t[k][0] = r0[c2];
-            t[k][1] = r1[c2];
-            t[k][2] = r0[c2b];
-            t[k][3] = r1[c2b];
+            f1 = __builtin_amdgcn_fmed3f(f1, -2.f, f1max);
+            f2 = __builtin_amdgcn_fmed3f(f2, -2.f, f2max);
+            w1[k] = __builtin_amdgcn_fractf(f1);
+            w2[k] = __builtin_amdgcn_fractf(f2);
+            const int j1 = int(floorf(f1)) + kJosephGuard, j2 = int(floorf(f2)) + kJosephGuard;
+            // slice base is wave-uniform (scalar), the in-slice byte offset a
+            // 32-bit lane value
+            const char* sl = reinterpret_cast<const char*>(base + size_t(s) * slice);
+            const unsigned o = unsigned(__mul24(j1, p2) + j2) * 4u;
+            const unsigned o2 = o + p2b;
+            ta[k] = *reinterpret_cast<const float2u*>(sl + o);
+            tb[k] = *reinterpret_cast<const float2u*>(sl + o2);
         }
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k)
-            acc += (wa[k] * t[k][0] + wb[k] * t[k][1]) + (wc[k] * t[k][2] + wd[k] * t[k][3]);
+        for (int k = 0; k < kUnroll; ++k) {
+            const float a = fmaf(w2[k], ta[k].y - ta[k].x, ta[k].x);
+            const float b = fmaf(w2[k], tb[k].y - tb[k].x, tb[k].x);
+            const float c = fmaf(w1[k], b - a, a);
+            acc += s0 + k <= whi ? c : 0.f;  // the tail repeats slice whi
+        }
     }
     if (!pix) return;
     const float out = lo <= hi ? acc * r.L : 0.f;
@@ -1095,6 +1108,26 @@ void launch_joseph_fp_sep(const JosephSepFpLaunch& p, hipStream_t s)
     hipLaunchKernelGGL(joseph_fp_sep_kernel, grid, block, 0, s, p.vol_zx, p.vol_zy, p.proj, p.views, p.vol,
         p.nu, p.nv, p.accumulate);
     YK_HIP(hipGetLastError());
+}
+
+void launch_transpose_zfast_guarded(const float* in, float* out, int nx, int ny, int nz, bool x_major,
+    hipStream_t s)
+{
+    const size_t p2 = size_t(nz) + 2 * kJosephGuard;
+    const size_t p1 = size_t(x_major ? ny : nx) + 2 * kJosephGuard;
+    launch_fill(out, 0.f, joseph_guarded_size(nx, ny, nz, x_major), s);
+    dim3 block(32, 8), grid((nx + 31) / 32, (nz + 31) / 32, ny);
+    const size_t sx = x_major ? p1 * p2 : p2;
+    const size_t sy = x_major ? p2 : p1 * p2;
+    hipLaunchKernelGGL(transpose_zfast_kernel, grid, block, 0, s, in, out + kJosephGuard * p2 + kJosephGuard,
+        nx, ny, nz, sx, sy);
+    YK_HIP(hipGetLastError());
+}
+
+size_t joseph_guarded_size(int nx, int ny, int nz, bool x_major)
+{
+    return size_t(x_major ? nx : ny) * (size_t(x_major ? ny : nx) + 2 * kJosephGuard) *
+        (size_t(nz) + 2 * kJosephGuard);
 }
 
 void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz, bool x_major, hipStream_t s)

@@ -9,6 +9,18 @@
 
 using namespace yk;
 
+namespace {
+
+// scratch that only grows: the separable paths alternate between layouts of
+// different sizes in one array, and a reallocation would synchronize
+template <typename T>
+void grow(DeviceArray<T>& a, size_t n)
+{
+    if (a.size() < n) a.resize(n);
+}
+
+}  // namespace
+
 yk_projector::yk_projector(yk_ctx* c, const yk_view* v, int n, int nu_, int nv_,
     const yk_volume_grid& g, float range)
     : ctx(c), nu(nu_), nv(nv_), n_views(n), grid(g),
@@ -116,14 +128,15 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
         any_y = any_y || ax == 1;
     }
     if (kind == YK_FP_JOSEPH && all_sep) {
-        const size_t nvox = size_t(grid.nx) * grid.ny * grid.nz;
+        // guarded copies (the Siddon path reuses d_vol_zx unguarded; every
+        // call rewrites the copy it reads)
         if (any_x) {
-            w.d_vol_zx.resize(nvox);
-            launch_transpose_zfast(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
+            grow(w.d_vol_zx, joseph_guarded_size(grid.nx, grid.ny, grid.nz, true));
+            launch_transpose_zfast_guarded(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
         }
         if (any_y) {
-            w.d_vol_zy.resize(nvox);
-            launch_transpose_zfast(d_volume, w.d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, s);
+            grow(w.d_vol_zy, joseph_guarded_size(grid.nx, grid.ny, grid.nz, false));
+            launch_transpose_zfast_guarded(d_volume, w.d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, s);
         }
         JosephSepFpLaunch p{};
         p.vol_zx = any_x ? w.d_vol_zx.get() : nullptr;
@@ -155,7 +168,7 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
     } else if (kind == YK_FP_SIDDON && std::all_of(siddon_separable.begin() + first,
                                                    siddon_separable.begin() + first + count,
                                                    [](char c) { return c != 0; })) {
-        w.d_vol_zx.resize(size_t(grid.nx) * grid.ny * grid.nz);
+        grow(w.d_vol_zx, size_t(grid.nx) * grid.ny * grid.nz);
         launch_transpose_zfast(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
         SiddonSepFpLaunch p{};
         p.vol_zx = w.d_vol_zx.get();

@@ -270,8 +270,8 @@ void launch_transpose_xy(const float* in, float* out, int nx, int ny, int nz, hi
 
 // Joseph projection for separable views from Z-fastest volume copies
 struct JosephSepFpLaunch {
-    const float* vol_zx;        // [x][y][z] copy (X-main views; may be null if none)
-    const float* vol_zy;        // [y][x][z] copy (Y-main views; may be null if none)
+    const float* vol_zx;        // guarded [x][y][z] copy (X-main views; may be null if none)
+    const float* vol_zy;        // guarded [y][x][z] copy (Y-main views; may be null if none)
     float* proj;                // [count][nv][nu]
     const ProjViewDev* views;   // device, `count`
     VolumeGeom vol;
@@ -281,6 +281,12 @@ struct JosephSepFpLaunch {
 void launch_joseph_fp_sep(const JosephSepFpLaunch& p, hipStream_t s);
 // [z][y][x] -> [x][y][z] (x_major) or [y][x][z]
 void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz, bool x_major, hipStream_t s);
+// the same copy with two zero planes before and after each minor axis (y or
+// x, and z), the layout joseph_fp_sep_kernel reads; joseph_guarded_size
+// floats
+void launch_transpose_zfast_guarded(const float* in, float* out, int nx, int ny, int nz, bool x_major,
+    hipStream_t s);
+size_t joseph_guarded_size(int nx, int ny, int nz, bool x_major);
 
 struct JosephAdjointLaunch {
     const float* proj;          // [count][nv][nu]
