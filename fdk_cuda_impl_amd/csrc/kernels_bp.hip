@@ -90,8 +90,11 @@ __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, Vol
     short4* __restrict__ tiles, uint2* __restrict__ stage, int* __restrict__ maxes)
 {
     const long long total = (long long)nbx * nby * nbz * n_views;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
+    const long long idx_raw = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane stays to the wave reduction below; lanes past the end
+    // evaluate entry 0 and store nothing
+    const bool valid = idx_raw < total;
+    const long long idx = valid ? idx_raw : 0;
     const int view = int(idx % n_views);
     const long long block = idx / n_views;
     const int ib = int(block % nbx);
@@ -117,6 +120,7 @@ __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, Vol
     }
     short4 t = make_short4(0, 0, 0, 0);
     uint2 st = make_uint2(0u, 0u);
+    int m_w = 0, m_h = 0, m_q = 0;
     if (!bad) {
         const double fu0 = floor(umin) - 1.0, fu1 = floor(umax) + 1.0;
         const double fv0 = floor(vmin) - 1.0, fv1 = floor(vmax) + 1.0;
@@ -131,9 +135,9 @@ __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, Vol
                 t.y = short(fv0);
                 t.z = short(fu1 - fu0 + 1.0);
                 t.w = short(fv1 - fv0 + 1.0);
-                atomicMax(&maxes[0], int(t.z));
-                atomicMax(&maxes[1], int(t.w));
-                atomicMax(&maxes[2], stage_quads(t.z, t.w));
+                m_w = int(t.z);
+                m_h = int(t.w);
+                m_q = stage_quads(t.z, t.w);
                 // staging shape, so the backprojection divides nothing per view:
                 // x = 16-bit reciprocal of the span, y = chunks | rows_per << 16
                 const StageShape sh = stage_shape(t.z, t.w);
@@ -142,7 +146,23 @@ __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, Vol
             }
         }
     }
-    if (bad) atomicMax(&maxes[3], 1);
+    // one atomic per wave and counter (a global atomic per entry serializes
+    // millions of updates on four addresses)
+    int m_bad = valid && bad ? 1 : 0;
+    if (!valid) m_w = m_h = m_q = 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        m_w = max(m_w, __shfl_xor(m_w, off, 64));
+        m_h = max(m_h, __shfl_xor(m_h, off, 64));
+        m_q = max(m_q, __shfl_xor(m_q, off, 64));
+        m_bad = max(m_bad, __shfl_xor(m_bad, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (m_w) atomicMax(&maxes[0], m_w);
+        if (m_h) atomicMax(&maxes[1], m_h);
+        if (m_q) atomicMax(&maxes[2], m_q);
+        if (m_bad) atomicMax(&maxes[3], 1);
+    }
+    if (!valid) return;
     tiles[idx] = t;
     stage[idx] = st;
 }
