@@ -267,6 +267,107 @@ joseph_fp_sep_kernel(const float* __restrict__ vzx, const float* __restrict__ vz
     *dst = accumulate ? *dst + out : out;
 }
 
+// Joseph projection for any X- or Y-main view set (tilted detectors, lifted
+// sources), one lane per ray as joseph_fp_kernel, over guarded slab copies:
+// [x][z][y] for X-main and [y][z][x] for Y-main views, with kJosephGuard
+// zero planes around both minor axes.  A slice of the main axis is one
+// contiguous slab (scalar base, 32-bit lane offsets); the first minor axis is
+// the fastest, so the lanes of a wave (consecutive u) read neighbouring
+// entries.  As in joseph_fp_sep_kernel the sample position is clamped to
+// [-2, n] and positions outside (-1, n) read only zeros; a sample is two
+// 8-byte loads and three lerps.
+__global__ void __launch_bounds__(256)
+joseph_fp_slab_kernel(const float* __restrict__ gx, const float* __restrict__ gy,
+    float* __restrict__ proj, const ProjViewDev* __restrict__ views, VolumeGeom g,
+    int nu, int nv, int accumulate)
+{
+    const int u = blockIdx.x * 64 + threadIdx.x;
+    const int v = blockIdx.y * 4 + threadIdx.y;
+    const int i = blockIdx.z;
+    const bool pix = u < nu && v < nv;
+    const ProjViewDev pv = views[i];
+    const AxisInfo ax = axis_info(pv.axis, g);
+    const float* base = pv.axis == 0 ? gx : gy;
+    JRay r{};
+    int lo = 1, hi = 0;
+    if (pix && joseph_ray(pv, ax, u, v, r)) joseph_slice_range(r, ax, lo, hi);
+    int wlo = lo <= hi ? lo : 0x7fffffff, whi = lo <= hi ? hi : -1;
+    for (int off = 32; off > 0; off >>= 1) {
+        wlo = min(wlo, __shfl_xor(wlo, off, 64));
+        whi = max(whi, __shfl_xor(whi, off, 64));
+    }
+    wlo = __builtin_amdgcn_readfirstlane(wlo);
+    whi = __builtin_amdgcn_readfirstlane(whi);
+    const int p1 = ax.n1 + 2 * kJosephGuard;
+    const unsigned p1b = unsigned(p1) * 4u;
+    const size_t slab = size_t(ax.n2 + 2 * kJosephGuard) * p1;
+    const float f1max = float(ax.n1), f2max = float(ax.n2);
+    float acc = 0.f;
+    constexpr int kUnroll = 4;
+    for (int s0 = wlo; s0 <= whi; s0 += kUnroll) {
+        float2u ta[kUnroll], tb[kUnroll];
+        float w1[kUnroll], w2[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+            const int s = min(s0 + k, whi);
+            float f1, f2;
+            joseph_sample(r, ax, s, f1, f2);
+            f1 = __builtin_amdgcn_fmed3f(f1, -2.f, f1max);
+            f2 = __builtin_amdgcn_fmed3f(f2, -2.f, f2max);
+            w1[k] = __builtin_amdgcn_fractf(f1);
+            w2[k] = __builtin_amdgcn_fractf(f2);
+            const int j1 = int(floorf(f1)) + kJosephGuard, j2 = int(floorf(f2)) + kJosephGuard;
+            const char* sl = reinterpret_cast<const char*>(base + size_t(s) * slab);
+            const unsigned o = unsigned(__mul24(j2, p1) + j1) * 4u;
+            const unsigned o2 = o + p1b;
+            ta[k] = *reinterpret_cast<const float2u*>(sl + o);
+            tb[k] = *reinterpret_cast<const float2u*>(sl + o2);
+        }
+#pragma unroll
+        for (int k = 0; k < kUnroll; ++k) {
+            const float a = fmaf(w1[k], ta[k].y - ta[k].x, ta[k].x);
+            const float b = fmaf(w1[k], tb[k].y - tb[k].x, tb[k].x);
+            const float c = fmaf(w2[k], b - a, a);
+            acc += s0 + k <= whi ? c : 0.f;
+        }
+    }
+    if (!pix) return;
+    const float out = lo <= hi ? acc * r.L : 0.f;
+    float* dst = proj + ((size_t)i * nv + v) * nu + u;
+    *dst = accumulate ? *dst + out : out;
+}
+
+// [z][y][x] -> guarded [y][z][x] slab copy (x stays fastest: a row copy)
+__global__ void slab_copy_y_kernel(const float* __restrict__ in, float* __restrict__ out,
+    int nx, int ny, int nz)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= nx) return;
+    const size_t p1 = size_t(nx) + 2 * kJosephGuard, p2 = size_t(nz) + 2 * kJosephGuard;
+    out[(size_t(y) * p2 + z + kJosephGuard) * p1 + x + kJosephGuard] = in[(size_t(z) * ny + y) * nx + x];
+}
+
+// [z][y][x] -> guarded [x][z][y] slab copy: an x-y transpose per z plane,
+// 32 x 32 tiles through LDS
+__global__ void slab_copy_x_kernel(const float* __restrict__ in, float* __restrict__ out,
+    int nx, int ny, int nz)
+{
+    __shared__ float tile[32][33];
+    const int z = blockIdx.z;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int y = y0 + k, x = x0 + threadIdx.x;
+        if (x < nx && y < ny) tile[k][threadIdx.x] = in[(size_t(z) * ny + y) * nx + x];
+    }
+    __syncthreads();
+    const size_t p1 = size_t(ny) + 2 * kJosephGuard, p2 = size_t(nz) + 2 * kJosephGuard;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int x = x0 + k, y = y0 + threadIdx.x;
+        if (x < nx && y < ny) out[(size_t(x) * p2 + z + kJosephGuard) * p1 + y + kJosephGuard] = tile[threadIdx.x][k];
+    }
+}
+
 // [z][y][x] -> Z-fastest copy: out[x * sx + y * sy + z], one y row per
 // blockIdx.z, 32 x 32 (x, z) tiles through LDS
 __global__ void transpose_zfast_kernel(const float* __restrict__ in, float* __restrict__ out,
@@ -1128,6 +1229,33 @@ size_t joseph_guarded_size(int nx, int ny, int nz, bool x_major)
 {
     return size_t(x_major ? nx : ny) * (size_t(x_major ? ny : nx) + 2 * kJosephGuard) *
         (size_t(nz) + 2 * kJosephGuard);
+}
+
+size_t joseph_slab_size(int nx, int ny, int nz, bool x_main)
+{
+    return size_t(x_main ? nx : ny) * (size_t(nz) + 2 * kJosephGuard) *
+        (size_t(x_main ? ny : nx) + 2 * kJosephGuard);
+}
+
+void launch_slab_copy(const float* in, float* out, int nx, int ny, int nz, bool x_main, hipStream_t s)
+{
+    launch_fill(out, 0.f, joseph_slab_size(nx, ny, nz, x_main), s);
+    if (x_main) {
+        dim3 block(32, 8), grid((nx + 31) / 32, (ny + 31) / 32, nz);
+        hipLaunchKernelGGL(slab_copy_x_kernel, grid, block, 0, s, in, out, nx, ny, nz);
+    } else {
+        dim3 block(256), grid((nx + 255) / 256, ny, nz);
+        hipLaunchKernelGGL(slab_copy_y_kernel, grid, block, 0, s, in, out, nx, ny, nz);
+    }
+    YK_HIP(hipGetLastError());
+}
+
+void launch_joseph_fp_slab(const JosephSepFpLaunch& p, hipStream_t s)
+{
+    dim3 block(64, 4), grid((p.nu + 63) / 64, (p.nv + 3) / 4, p.count);
+    hipLaunchKernelGGL(joseph_fp_slab_kernel, grid, block, 0, s, p.vol_zx, p.vol_zy, p.proj, p.views, p.vol,
+        p.nu, p.nv, p.accumulate);
+    YK_HIP(hipGetLastError());
 }
 
 void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz, bool x_major, hipStream_t s)

@@ -120,12 +120,13 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
     if (!d_volume || !d_proj) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
     hipStream_t s = ctx->stream;
     yk_projector& w = scratch_owner ? *scratch_owner : *this;  // scratch arrays
-    bool all_sep = true, any_x = false, any_y = false;
+    bool all_sep = true, any_x = false, any_y = false, any_z = false;
     for (int i = first; i < first + count; ++i) {
         all_sep = all_sep && separable[i];
         const int ax = main_axis(views[i], nu, nv);
         any_x = any_x || ax == 0;
         any_y = any_y || ax == 1;
+        any_z = any_z || ax == 2;
     }
     if (kind == YK_FP_JOSEPH && all_sep) {
         // guarded copies (the Siddon path reuses d_vol_zx unguarded; every
@@ -147,6 +148,25 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
         p.nu = nu; p.nv = nv; p.count = count;
         p.accumulate = accumulate;
         launch_joseph_fp_sep(p, s);
+    } else if (kind == YK_FP_JOSEPH && !any_z) {
+        // X/Y-main views of any orientation: guarded slab copies
+        if (any_x) {
+            grow(w.d_vol_zx, joseph_slab_size(grid.nx, grid.ny, grid.nz, true));
+            launch_slab_copy(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
+        }
+        if (any_y) {
+            grow(w.d_vol_zy, joseph_slab_size(grid.nx, grid.ny, grid.nz, false));
+            launch_slab_copy(d_volume, w.d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, s);
+        }
+        JosephSepFpLaunch p{};
+        p.vol_zx = any_x ? w.d_vol_zx.get() : nullptr;
+        p.vol_zy = any_y ? w.d_vol_zy.get() : nullptr;
+        p.proj = d_proj;
+        p.views = d_jviews.get() + first;
+        p.vol = vol;
+        p.nu = nu; p.nv = nv; p.count = count;
+        p.accumulate = accumulate;
+        launch_joseph_fp_slab(p, s);
     } else if (kind == YK_FP_JOSEPH) {
         bool need_t = false;
         std::vector<ProjViewDev> dummy;

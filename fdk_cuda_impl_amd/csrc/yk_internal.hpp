@@ -287,6 +287,12 @@ void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz,
 void launch_transpose_zfast_guarded(const float* in, float* out, int nx, int ny, int nz, bool x_major,
     hipStream_t s);
 size_t joseph_guarded_size(int nx, int ny, int nz, bool x_major);
+// Joseph projection of X- or Y-main views of any orientation from guarded
+// slab copies ([x][z][y] in vol_zx, [y][z][x] in vol_zy; joseph_slab_size
+// floats each, made by launch_slab_copy)
+void launch_joseph_fp_slab(const JosephSepFpLaunch& p, hipStream_t s);
+void launch_slab_copy(const float* in, float* out, int nx, int ny, int nz, bool x_main, hipStream_t s);
+size_t joseph_slab_size(int nx, int ny, int nz, bool x_main);
 
 struct JosephAdjointLaunch {
     const float* proj;          // [count][nv][nu]
