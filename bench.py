@@ -96,6 +96,15 @@ def projector_leg(yk, phantoms, ctx, stream):
         out[f"{kind}_adjoint_ms"] = round(time_launches(lambda: P.adjoint(proj, kind=kind, out=back), 3, 1,
                                                         stream), 3)
     P.close()
+    # the general (non-separable) kernels: the same scan with the detector
+    # tilted by (2, -1, 3) mrad about U, V, N
+    tgeom = yk.ScanGeometry(360, 360, 500.0, 1000.0, 512, 384, 0.6, 0.6, volume=(256, 256, 256),
+                            voxel_mm=(0.5, 0.5, 0.5), tilt_rad=(0.002, -0.001, 0.003))
+    PT = yk.Projector(ctx, tgeom.views(), tgeom.nu, tgeom.nv, grid)
+    for kind in ("joseph", "siddon"):
+        out[f"tilted_{kind}_fp_ms"] = round(time_launches(lambda: PT.forward(vol, kind=kind, out=proj), 3, 1,
+                                                          stream), 3)
+    PT.close()
     samples = 360 * geom.nu * geom.nv * 256
     out["joseph_fp_gsamples_per_s"] = round(samples / (out["joseph_fp_ms"] * 1e-3) / 1e9, 1)
     out["workload"] = "256^3 @ 0.5 mm, 360 views, 512x384 @ 0.6 mm, SOD 500 / SDD 1000"
