@@ -125,6 +125,61 @@ __device__ float2* fft_forward(float2* a, float2* b, const float2* tw, int n)
     return src;
 }
 
+// The same transform for a length known at compile time (n = NPT x the
+// workgroup size, every length of 256 and above): the pass sequence, spans
+// and lane loops are constants, so the index arithmetic folds to shifts and
+// masks, and the first pass (span 1, all twiddles 1) multiplies by nothing.
+// The operations on the data are those of fft_forward.
+template <int R, int N, int NS>
+__device__ __forceinline__ void stockham_pass_ct(const float2* src, float2* dst, const float2* tw)
+{
+    constexpr int m = N / R;
+#pragma unroll 1
+    for (int j0 = 0; j0 < m; j0 += kFilterThreads) {
+        const int j = j0 + int(threadIdx.x);
+        if (m % kFilterThreads != 0 && j >= m) break;
+        const int k = j & (NS - 1);
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = src[pidx(j + r * m)];
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[(r - 1) * NS + k]);
+        }
+        dft<R>(v);
+        const int idx = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[pidx(idx + r * NS)] = v[r];
+    }
+}
+
+template <int N, int NS>
+__device__ __forceinline__ float2* fft_forward_ct(float2* src, float2* dst, const float2* tw)
+{
+    if constexpr (NS >= N) {
+        (void)dst; (void)tw;
+        return src;
+    } else {
+        constexpr int R = NS * 8 <= N ? 8 : (NS * 4 <= N ? 4 : 2);
+        stockham_pass_ct<R, N, NS>(src, dst, tw);
+        __syncthreads();
+        return fft_forward_ct<N, NS * R>(dst, src, tw + (R - 1) * NS);
+    }
+}
+
+// NPT > 1 means n = NPT x the workgroup size exactly (the launcher picks
+// NPT = n / 256 for power-of-two n >= 512); NPT = 1 also serves n < 256
+template <int NPT>
+__device__ __forceinline__ float2* fft_rows(float2* a, float2* b, const float2* tw, int n)
+{
+    if constexpr (NPT > 1) {
+        (void)n;
+        return fft_forward_ct<NPT * kFilterThreads, 1>(a, b, tw);
+    } else {
+        return fft_forward(a, b, tw, n);
+    }
+}
+
 // cosine weight SDD_r / |q| * SDD_r / (2 SOD), q = detS + u detU + v detV - src
 // (.../FDK/kernels/YkFDKPreWeightKernels.cu:374,401-402), times Parker
 __device__ __forceinline__ float row_weight(const FilterView& fv, const float* __restrict__ parker_row,
@@ -244,7 +299,7 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
         __syncthreads();
         if (p + int(gridDim.x) < pairs) prefetch(p + gridDim.x);
 
-        float2* spec = fft_forward(bufa, bufb, tws, n);
+        float2* spec = fft_rows<NPT>(bufa, bufb, tws, n);
         // Y = conj(X W): the second forward transform then yields conj(IFFT(X W))
         for (int k = threadIdx.x; k < n; k += kFilterThreads) {
             const float w = weights[k <= n / 2 ? k : n - k];
@@ -253,7 +308,7 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
         }
         __syncthreads();
         float2* other = spec == bufa ? bufb : bufa;
-        float2* res = fft_forward(spec, other, tws, n);
+        float2* res = fft_rows<NPT>(spec, other, tws, n);
 
         // crop: row a = Re(res), row b = -Im(res)
         float inv_a = 1.f, inv_b = 1.f;
