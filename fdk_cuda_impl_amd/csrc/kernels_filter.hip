@@ -130,8 +130,9 @@ __device__ float2* fft_forward(float2* a, float2* b, const float2* tw, int n)
 // and lane loops are constants, so the index arithmetic folds to shifts and
 // masks, and the first pass (span 1, all twiddles 1) multiplies by nothing.
 // The operations on the data are those of fft_forward.
-template <int R, int N, int NS>
-__device__ __forceinline__ void stockham_pass_ct(const float2* src, float2* dst, const float2* tw)
+template <int R, int N, int NS, bool SPECTRAL = false>
+__device__ __forceinline__ void stockham_pass_ct(const float2* src, float2* dst, const float2* tw,
+    const float* __restrict__ weights = nullptr)
 {
     constexpr int m = N / R;
 #pragma unroll 1
@@ -149,21 +150,31 @@ __device__ __forceinline__ void stockham_pass_ct(const float2* src, float2* dst,
         dft<R>(v);
         const int idx = (j - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[pidx(idx + r * NS)] = v[r];
+        for (int r = 0; r < R; ++r) {
+            if constexpr (SPECTRAL) {
+                // last pass: bin idx + r NS in natural order; Y = conj(X W)
+                const int b = idx + r * NS;
+                const float w = weights[b <= N / 2 ? b : N - b];
+                v[r] = make_float2(v[r].x * w, -v[r].y * w);
+            }
+            dst[pidx(idx + r * NS)] = v[r];
+        }
     }
 }
 
-template <int N, int NS>
-__device__ __forceinline__ float2* fft_forward_ct(float2* src, float2* dst, const float2* tw)
+// SPECTRAL: the last pass also applies the filter weights and conjugates
+template <int N, int NS, bool SPECTRAL>
+__device__ __forceinline__ float2* fft_forward_ct(float2* src, float2* dst, const float2* tw,
+    const float* __restrict__ weights)
 {
     if constexpr (NS >= N) {
-        (void)dst; (void)tw;
+        (void)dst; (void)tw; (void)weights;
         return src;
     } else {
         constexpr int R = NS * 8 <= N ? 8 : (NS * 4 <= N ? 4 : 2);
-        stockham_pass_ct<R, N, NS>(src, dst, tw);
+        stockham_pass_ct<R, N, NS, SPECTRAL && NS * R >= N>(src, dst, tw, weights);
         __syncthreads();
-        return fft_forward_ct<N, NS * R>(dst, src, tw + (R - 1) * NS);
+        return fft_forward_ct<N, NS * R, SPECTRAL>(dst, src, tw + (R - 1) * NS, weights);
     }
 }
 
@@ -174,7 +185,7 @@ __device__ __forceinline__ float2* fft_rows(float2* a, float2* b, const float2* 
 {
     if constexpr (NPT > 1) {
         (void)n;
-        return fft_forward_ct<NPT * kFilterThreads, 1>(a, b, tw);
+        return fft_forward_ct<NPT * kFilterThreads, 1, false>(a, b, tw, nullptr);
     } else {
         return fft_forward(a, b, tw, n);
     }
@@ -299,14 +310,19 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
         __syncthreads();
         if (p + int(gridDim.x) < pairs) prefetch(p + gridDim.x);
 
-        float2* spec = fft_rows<NPT>(bufa, bufb, tws, n);
         // Y = conj(X W): the second forward transform then yields conj(IFFT(X W))
-        for (int k = threadIdx.x; k < n; k += kFilterThreads) {
-            const float w = weights[k <= n / 2 ? k : n - k];
-            const float2 x = spec[pidx(k)];
-            spec[pidx(k)] = make_float2(x.x * w, -x.y * w);
+        float2* spec;
+        if constexpr (NPT > 1) {
+            spec = fft_forward_ct<NPT * kFilterThreads, 1, true>(bufa, bufb, tws, weights);
+        } else {
+            spec = fft_forward(bufa, bufb, tws, n);
+            for (int k = threadIdx.x; k < n; k += kFilterThreads) {
+                const float w = weights[k <= n / 2 ? k : n - k];
+                const float2 x = spec[pidx(k)];
+                spec[pidx(k)] = make_float2(x.x * w, -x.y * w);
+            }
+            __syncthreads();
         }
-        __syncthreads();
         float2* other = spec == bufa ? bufb : bufa;
         float2* res = fft_rows<NPT>(spec, other, tws, n);
 
