@@ -296,32 +296,61 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
         const FilterView fb = views[B.view];
         const float* park_a = parker ? parker + (size_t)A.view * nu : nullptr;
         const float* park_b = parker ? parker + (size_t)B.view * nu : nullptr;
-        // cosine (and Parker) weights on the way into LDS
+        // Y = conj(X W): the second forward transform then yields conj(IFFT(X W))
+        float2* spec;
+        if constexpr (NPT >= 8) {
+            // The first radix-8 pass (span 1, no twiddles) reads elements
+            // j + r N/8, which for N/8 a multiple of the workgroup size are
+            // this lane's own prefetched texels: weight them (cosine and
+            // Parker) in registers and transform them straight into LDS.
+            constexpr int N = NPT * kFilterThreads, M = N / 8, S = M / kFilterThreads;
+            float2 wv[NPT];
 #pragma unroll
-        for (int e = 0; e < NPT; ++e) {
-            const int m = int(threadIdx.x) + e * kFilterThreads;
-            if (m < n) {
+            for (int e = 0; e < NPT; ++e) {
+                const int m = int(threadIdx.x) + e * kFilterThreads;
                 const int ua = m - fa.start_u, ub = m - fb.start_u;
                 const float wa = (ua >= 0 && ua < nu) ? row_weight(fa, park_a, A.v, ua) : 0.f;
                 const float wb = (has_b && ub >= 0 && ub < nu) ? row_weight(fb, park_b, B.v, ub) : 0.f;
-                bufa[pidx(m)] = make_float2(xa[e] * wa, xb[e] * wb);
+                wv[e] = make_float2(xa[e] * wa, xb[e] * wb);
             }
-        }
-        __syncthreads();
-        if (p + int(gridDim.x) < pairs) prefetch(p + gridDim.x);
-
-        // Y = conj(X W): the second forward transform then yields conj(IFFT(X W))
-        float2* spec;
-        if constexpr (NPT > 1) {
-            spec = fft_forward_ct<NPT * kFilterThreads, 1, true>(bufa, bufb, tws, weights);
-        } else {
-            spec = fft_forward(bufa, bufb, tws, n);
-            for (int k = threadIdx.x; k < n; k += kFilterThreads) {
-                const float w = weights[k <= n / 2 ? k : n - k];
-                const float2 x = spec[pidx(k)];
-                spec[pidx(k)] = make_float2(x.x * w, -x.y * w);
+#pragma unroll
+            for (int t = 0; t < S; ++t) {
+                float2 v[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) v[r] = wv[t + r * S];
+                dft<8>(v);
+                const int j = t * kFilterThreads + int(threadIdx.x);
+#pragma unroll
+                for (int r = 0; r < 8; ++r) bufb[pidx(j * 8 + r)] = v[r];
             }
             __syncthreads();
+            if (p + int(gridDim.x) < pairs) prefetch(p + gridDim.x);
+            spec = fft_forward_ct<N, 8, true>(bufb, bufa, tws + 7, weights);
+        } else {
+            // cosine (and Parker) weights on the way into LDS
+#pragma unroll
+            for (int e = 0; e < NPT; ++e) {
+                const int m = int(threadIdx.x) + e * kFilterThreads;
+                if (m < n) {
+                    const int ua = m - fa.start_u, ub = m - fb.start_u;
+                    const float wa = (ua >= 0 && ua < nu) ? row_weight(fa, park_a, A.v, ua) : 0.f;
+                    const float wb = (has_b && ub >= 0 && ub < nu) ? row_weight(fb, park_b, B.v, ub) : 0.f;
+                    bufa[pidx(m)] = make_float2(xa[e] * wa, xb[e] * wb);
+                }
+            }
+            __syncthreads();
+            if (p + int(gridDim.x) < pairs) prefetch(p + gridDim.x);
+            if constexpr (NPT > 1) {
+                spec = fft_forward_ct<NPT * kFilterThreads, 1, true>(bufa, bufb, tws, weights);
+            } else {
+                spec = fft_forward(bufa, bufb, tws, n);
+                for (int k = threadIdx.x; k < n; k += kFilterThreads) {
+                    const float w = weights[k <= n / 2 ? k : n - k];
+                    const float2 x = spec[pidx(k)];
+                    spec[pidx(k)] = make_float2(x.x * w, -x.y * w);
+                }
+                __syncthreads();
+            }
         }
         float2* other = spec == bufa ? bufb : bufa;
         float2* res = fft_rows<NPT>(spec, other, tws, n);
