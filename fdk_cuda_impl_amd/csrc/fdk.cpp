@@ -95,6 +95,8 @@ yk_fdk::yk_fdk(yk_ctx* c, const yk_fdk_desc& d, const yk_view* v, int n, int nu_
 {
     if (!v || n <= 0) fail(YK_ERR_INVALID_GEOMETRY, "FDK needs at least one view");
     if (nu <= 0 || nv <= 0) fail(YK_ERR_INVALID_GEOMETRY, "invalid detector size");
+    // the backprojection stages texel pairs (c, c + 1) of a row
+    if (nu < 2) fail(YK_ERR_UNSUPPORTED, "FDK needs a detector of at least two channels");
     validate_volume(grid);
     validate_filter(desc.filter);
     storage_bytes(desc.storage);

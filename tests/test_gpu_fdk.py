@@ -41,6 +41,46 @@ def test_filter_stage_matches_oracle(ctx, oracle, case):
     assert max_abs(got, ref) < 1e-5 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("nu", [256, 512, 1024, 1536])
+@pytest.mark.parametrize("case", ["full", "short-offset"])
+def test_filter_stage_lengths_match_oracle(ctx, oracle, nu, case):
+    """The filter kernel's compile-time lengths N = 512, 1024, 2048 and 4096
+    (NPT = 2, 4, 8 and 16 points per lane; the register first pass from
+    N = 2048), with FP32 and FP16 outputs, against the oracle's FP64 rows.
+    "short-offset": a 234-degree short scan with Parker weights, a 2.5-pixel
+    detector offset (the row is not centred in its padded length) and an odd
+    row count."""
+    kw = dict(channels=nu, rows=7, channel_mm=0.1, row_mm=0.1, volume=(24, 20, 18),
+              voxel_mm=(0.05, 0.05, 0.05))
+    if case == "full":
+        kw.update(total_views=3, views_per_turn=24)
+        parker = None
+    else:
+        kw.update(total_views=13, views_per_turn=20, start_angle_rad=0.3, offset_unv_mm=(0.25, 0.0, 0.0))
+        parker = (0.3, 1)
+    g = small_geometry(**kw)
+    plan = yk.FdkPlan.from_geometry(ctx, g, filter="hann", parker="auto" if parker else "disabled")
+    assert plan.padded_n == {256: 512, 512: 1024, 1024: 2048, 1536: 4096}[nu]
+    proj = seeded_projection(11, g.total_views, g.nv, g.nu)
+    ref, _ = oracle_filtered(oracle, plan, g, proj, filt="hann", parker=parker)
+    dproj = torch.from_numpy(proj).cuda()
+    out32, _ = plan.filter(dproj, storage="f32")
+    got = out32.cpu().numpy()
+    assert rel_rmse(got, ref) < 2e-6, rel_rmse(got, ref)
+    assert max_abs(got, ref) < 1e-5 * np.abs(ref).max()
+    out16, scales = plan.filter(dproj, storage="f16")
+    got16 = out16.float().cpu().numpy() * scales.cpu().numpy()[:, :, None]
+    # rows a Parker weight zeroes (or nearly: FP32 and FP64 angles differ at the
+    # ends of the short scan) are measured against the global peak
+    row_peak = np.maximum(np.abs(ref).max(axis=2), 1e-3 * np.abs(ref).max())
+    row_err = np.abs(got16 - ref).max(axis=2) / row_peak
+    print(f"filter nu={nu} ({case}): FP32 rel-RMSE {rel_rmse(got, ref):.2e}, FP16 rel-RMSE "
+          f"{rel_rmse(got16, ref):.2e}, worst FP16 row error / row peak {row_err.max():.2e}")
+    assert rel_rmse(got16, ref) < 3e-4
+    assert row_err.max() < 2.0 ** -10
+    plan.close()
+
+
 @pytest.mark.parametrize("kw", [
     dict(),
     dict(offset_unv_mm=(2.5, 0.0, -1.5), source_offset_mm=(0.0, 0.0, 2.0), center_mm=(1.0, -2.0, 0.5)),
@@ -205,3 +245,12 @@ def test_reference_shepp_logan_quality(ctx):
     cos = t @ rec / np.sqrt((t @ t) * (rec @ rec))
     nrmse = np.linalg.norm(rec - t) / np.linalg.norm(t)
     assert cos > 0.85 and nrmse < 0.65, (cos, nrmse)
+
+
+def test_single_channel_detector_is_refused(ctx):
+    """The backprojection stages texel pairs of a row, so an FDK plan needs at
+    least two detector channels; one channel is UnsupportedCombination."""
+    g = small_geometry(channels=1)
+    with pytest.raises(yk.YkError) as e:
+        yk.FdkPlan.from_geometry(ctx, g)
+    assert e.value.status == 3  # UnsupportedCombination
