@@ -49,6 +49,7 @@ constexpr int kBY = 16;
 constexpr int kThreads = kBX * kBY;
 constexpr int kKahanViews = 64;
 constexpr int kMaxRows = 12;  // texel rows a thread prefetches (quad rows + 1)
+constexpr double kFootMargin = 1.0 / 32.0;  // pixels of margin around a block's footprint
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
@@ -71,19 +72,17 @@ __host__ __device__ __forceinline__ StageShape stage_shape(int qw, int qh)
 // neighbouring rows fall on different banks
 __host__ __device__ __forceinline__ int lds_pitch(int qw) { return qw | 1; }
 
-// quads of LDS a tile needs: the staging writes whole row ranges, so the
-// last range may spill past qh rows
-__host__ __device__ __forceinline__ int stage_quads(int qw, int qh)
-{
-    const StageShape s = stage_shape(qw, qh);
-    const bool fast = qw <= kThreads && s.rows_per + 1 <= kMaxRows;
-    return lds_pitch(qw) * (fast ? s.chunks * s.rows_per : qh);
-}
+// quads of LDS a tile needs: the staging writes rows [0, qh) only (the last
+// row range is moved back to end at qh, see stage_map)
+__host__ __device__ __forceinline__ int stage_quads(int qw, int qh) { return lds_pitch(qw) * qh; }
 
 // ---------------------------------------------------------------------------
 // footprint table: per (block, view) the quad range [u0, u0+w) x [v0, v0+h)
-// that contains floor(u), floor(v) of every voxel of the block, with one quad
-// of margin on each side; w = h = 0 when the block misses the detector.
+// that contains floor(u), floor(v) of every voxel of the block, widened by
+// kFootMargin pixels on each side; w = h = 0 when the block misses the
+// detector.  The kernel's FP32 u, v stay within ~1e-3 pixel of the FP64
+// values here (FP32 coefficients, one reciprocal, at most 32 incremental
+// steps), so a 1/32-pixel margin keeps every sample inside its tile.
 // ---------------------------------------------------------------------------
 __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, VolumeGeom vol,
     int bx, int by, int bz, int nbx, int nby, int nbz, int nu, int nv,
@@ -122,8 +121,8 @@ __global__ void bp_tiles_kernel(const double* __restrict__ c64, int n_views, Vol
     uint2 st = make_uint2(0u, 0u);
     int m_w = 0, m_h = 0, m_q = 0;
     if (!bad) {
-        const double fu0 = floor(umin) - 1.0, fu1 = floor(umax) + 1.0;
-        const double fv0 = floor(vmin) - 1.0, fv1 = floor(vmax) + 1.0;
+        const double fu0 = floor(umin - kFootMargin), fu1 = floor(umax + kFootMargin);
+        const double fv0 = floor(vmin - kFootMargin), fv1 = floor(vmax + kFootMargin);
         // quads with index in [-1, n-1] can touch a texel
         const bool miss = fu1 < -1.0 || fu0 > nu - 1.0 || fv1 < -1.0 || fv0 > nv - 1.0;
         if (!miss) {
@@ -258,7 +257,7 @@ struct StageMap {
     int col;      // quad column of this lane within the tile
     int cl;       // first loaded texel column, clamped into [0, nu - 2]
     int shift;    // requested - loaded first column: -1, 0 or 1 on the detector edges
-    int r0;       // first quad row of this lane
+    int r0;       // first quad row of this lane (rows [r0, r0 + rows_per) lie in [0, qh))
     bool lane_on; // lane has a row range
     bool fast;    // tile fits the register prefetch
 };
@@ -271,12 +270,14 @@ __device__ __forceinline__ StageMap stage_map(short4 t, uint2 st, int nu)
     m.s.rows_per = __builtin_amdgcn_readfirstlane(int(st.y >> 16));
     // threadIdx.x / span by the 16-bit reciprocal st.x: exact for
     // threadIdx.x < 256 and span <= 256 (the rounding excess stays below 1/256)
-    const int rc = int((threadIdx.x * st.x) >> 16);
-    m.col = int(threadIdx.x) - rc * m.s.span;
+    const int rc = int(__umul24(threadIdx.x, st.x) >> 16);
+    m.col = int(threadIdx.x) - __mul24(rc, m.s.span);
     const int col = t.x + m.col;
     m.cl = min(max(col, 0), nu - 2);
     m.shift = col - m.cl;
-    m.r0 = rc * m.s.rows_per;
+    // the last row range ends at qh: it overlaps its neighbour (the rows
+    // written twice get the same values) instead of spilling past the tile
+    m.r0 = min(__mul24(rc, m.s.rows_per), int(t.w) - m.s.rows_per);
     m.lane_on = rc < m.s.chunks;
     m.fast = t.z <= kThreads && m.s.rows_per + 1 <= kMaxRows;
     return m;
@@ -334,7 +335,10 @@ __device__ __forceinline__ void prefetch_tile(Prefetch<STORAGE>& pf, const Stage
     if (!m.lane_on) return;
     constexpr int eb = STORAGE == YK_F32 ? 4 : 2;
     const int row0 = t.y + m.r0;
-    const int off0 = (row0 * nu + m.cl) * eb;
+    // byte offset of the first row; the empty asm keeps the row steps below
+    // from being folded back into a per-row multiply
+    int off0 = (__mul24(row0, nu) + m.cl) * eb;
+    asm volatile("" : "+v"(off0));
     const int rowb = nu * eb;
     const int rows_per = m.s.rows_per;  // uniform across the workgroup
 #pragma unroll
@@ -377,8 +381,7 @@ __device__ __forceinline__ half2_t to_half2(unsigned raw, unsigned f)
     }
 }
 
-// Write quad rows [r0, r0 + rows_per) of this lane's column.  Rows past the
-// tile land in the LDS padding (stage_quads) and are never read.
+// Write quad rows [r0, r0 + rows_per) of this lane's column.
 template <int STORAGE>
 __device__ __forceinline__ void commit_tile(void* lds, Prefetch<STORAGE>& pf, const StageMap& m, short4 t)
 {
@@ -391,7 +394,7 @@ __device__ __forceinline__ void commit_tile(void* lds, Prefetch<STORAGE>& pf, co
     }
     constexpr int qb = STORAGE == YK_F32 ? 16 : 8;
     const int pitch = lds_pitch(t.z);
-    char* dst = static_cast<char*>(lds) + (m.r0 * pitch + m.col) * qb;
+    char* dst = static_cast<char*>(lds) + unsigned(__mul24(m.r0, pitch) + m.col) * unsigned(qb);
     const int rowb = pitch * qb;
     if constexpr (STORAGE == YK_F32) {
 #pragma unroll
