@@ -25,6 +25,8 @@ BP_JOSEPH_ADJOINT, BP_SIDDON_ADJOINT, BP_FDK, BP_FDK_PSEUDO_MATCHED, BP_JOSEPH_V
 PARKER_AUTO, PARKER_DISABLED, PARKER_ENABLED = 0, 1, 2
 FILTERS = {"none": 0, "ramlak": 1, "shepplogan": 2, "cosine": 3, "hann": 4, "hamming": 5,
            "blackman": 6, "butterworth": 7, "kaiser": 8, "tukey": 9}
+# yk_progress_fn: (user data, completed, total)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int)
 STORAGE = {"f32": F32, "float32": F32, "f16": F16, "float16": F16, "bf16": BF16, "bfloat16": BF16}
 
 
@@ -98,7 +100,8 @@ class AlgebraicDesc(ctypes.Structure):
                 ("relative_update_tolerance", ctypes.c_float),
                 ("relative_improvement_tolerance", ctypes.c_float),
                 ("minimum_iterations", ctypes.c_int), ("check_interval", ctypes.c_int),
-                ("patience", ctypes.c_int), ("tv_dimensions", ctypes.c_int)]
+                ("patience", ctypes.c_int), ("tv_dimensions", ctypes.c_int),
+                ("progress_callback", ctypes.c_void_p), ("progress_user_data", ctypes.c_void_p)]
 
 
 class FdkStage(ctypes.Structure):
@@ -132,7 +135,8 @@ class CglsDesc(ctypes.Structure):
                 ("relative_update_tolerance", ctypes.c_float),
                 ("relative_improvement_tolerance", ctypes.c_float),
                 ("minimum_iterations", ctypes.c_int), ("check_interval", ctypes.c_int),
-                ("patience", ctypes.c_int)]
+                ("patience", ctypes.c_int),
+                ("progress_callback", ctypes.c_void_p), ("progress_user_data", ctypes.c_void_p)]
 
 
 class IterativeStats(ctypes.Structure):

@@ -233,6 +233,7 @@ void run_algebraic(yk_projector* p, const yk_algebraic_desc& d, const float* d_b
         }
         if (d.use_min || d.use_max) launch_bounds_clamp(d_x, d.use_min, d.min_value, d.use_max, d.max_value, nx, s);
         st.completed_iterations = outer + 1;
+        report_progress(d.progress_callback, d.progress_user_data, outer + 1, d.iterations, s);
         // shouldCheckConvergence_ / checkConvergence_ (YkAlgebraicReconstructorEx.hpp:258-330)
         if ((!watch_residual && !watch_update) ||
             ((outer + 1) % d.check_interval != 0 && outer + 1 != d.iterations))

@@ -24,6 +24,14 @@ void check_hip(hipError_t err, const char* what)
         throw Error(YK_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(err));
 }
 
+void report_progress(yk_progress_fn cb, void* user, int completed, int total, hipStream_t s)
+{
+    if (!cb) return;
+    // the report describes finished GPU work, not enqueued work
+    YK_HIP(hipStreamSynchronize(s));
+    cb(user, completed, total);
+}
+
 void PinnedBuffer::ensure(size_t bytes)
 {
     if (bytes <= bytes_) return;
@@ -552,10 +560,14 @@ yk_status yk_cgls_ex(yk_projector* p, const yk_cgls_desc* desc, const float* d_b
 {
     return guard([&] {
         require(p != nullptr && desc != nullptr, YK_ERR_INVALID_CONFIG, "null projector or descriptor");
-        require(desc->struct_size >= sizeof(yk_cgls_desc), YK_ERR_INVALID_CONFIG,
+        require(desc->struct_size >= offsetof(yk_cgls_desc, progress_callback), YK_ERR_INVALID_CONFIG,
             "yk_cgls_desc is truncated or from an older layout");
         require(d_b && d_x, YK_ERR_INVALID_BUFFER, "null device buffer");
-        run_cgls(p, *desc, d_b, d_x, stats, residual_norms);
+        // an older caller's struct ends before the progress tail: read only
+        // struct_size bytes of it
+        yk_cgls_desc d{};
+        std::memcpy(&d, desc, std::min<size_t>(desc->struct_size, sizeof(d)));
+        run_cgls(p, d, d_b, d_x, stats, residual_norms);
         p->ctx->finish();
     });
 }
@@ -597,10 +609,12 @@ yk_status yk_algebraic(yk_projector* p, const yk_algebraic_desc* desc, const flo
 {
     return guard([&] {
         require(p != nullptr && desc != nullptr, YK_ERR_INVALID_CONFIG, "null projector or descriptor");
-        require(desc->struct_size >= sizeof(yk_algebraic_desc), YK_ERR_INVALID_CONFIG,
+        require(desc->struct_size >= offsetof(yk_algebraic_desc, progress_callback), YK_ERR_INVALID_CONFIG,
             "yk_algebraic_desc is truncated or from an older layout");
         require(d_b && d_x, YK_ERR_INVALID_BUFFER, "null device buffer");
-        run_algebraic(p, *desc, d_b, d_x, stats);
+        yk_algebraic_desc d{};
+        std::memcpy(&d, desc, std::min<size_t>(desc->struct_size, sizeof(d)));
+        run_algebraic(p, d, d_b, d_x, stats);
         p->ctx->finish();
     });
 }

@@ -99,6 +99,7 @@ void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d
             bounds();
             st.completed_iterations = k + 1;
             if (residual_norms) residual_norms[k + 1] = std::sqrt(p->dot(v.r.get(), v.r.get(), np));
+            report_progress(d.progress_callback, d.progress_user_data, k + 1, d.iterations, s);
         }
         for (int j = k; j < d.iterations && residual_norms; ++j) residual_norms[j + 1] = residual_norms[j];
     } else {
@@ -149,6 +150,7 @@ void run_cgls(yk_projector* p, const yk_cgls_desc& d, const float* d_b, float* d
             bounds();
             st.completed_iterations = k + 1;
             if (residual_norms) residual_norms[k + 1] = l2;
+            report_progress(d.progress_callback, d.progress_user_data, k + 1, d.iterations, s);
             // checkConvergence_ (YkCglsBackends.hpp:454-500)
             if (!convergence_enabled(d) ||
                 (st.completed_iterations % d.check_interval != 0 && st.completed_iterations != d.iterations))

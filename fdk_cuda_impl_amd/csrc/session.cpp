@@ -72,6 +72,82 @@ void rotate_views(std::vector<yk_view>& views, const SForwardProjectionPose& pos
     }
 }
 
+// The flat circular part of SSystemConfig as the C ABI's yk_system.
+yk_system c_system(const SSystemConfig& g)
+{
+    yk_system sys{};
+    const SRegularCircularScanSpec& c = g.circular;
+    sys.scan.total_views = c.total_views;
+    sys.scan.views_per_turn = c.views_per_turn;
+    sys.scan.start_angle_rad = c.start_angle_rad;
+    sys.scan.rotation_direction = c.rotation_direction;
+    sys.scan.sid_mm = c.sid_mm;
+    sys.scan.sdd_mm = c.sdd_mm;
+    sys.scan.z_mm = c.z_mm;
+    sys.scan.source_offset_mm[0] = c.source_offset_mm.x;
+    sys.scan.source_offset_mm[1] = c.source_offset_mm.y;
+    sys.scan.source_offset_mm[2] = c.source_offset_mm.z;
+    const SFlatDetectorSpec& d = g.flat_detector;
+    sys.detector.channels = d.channels;
+    sys.detector.rows = d.rows;
+    sys.detector.channel_size_mm = d.channel_size_mm;
+    sys.detector.row_size_mm = d.row_size_mm;
+    sys.detector.pose.offset_unv_mm[0] = d.pose.offset_unv_mm.x;
+    sys.detector.pose.offset_unv_mm[1] = d.pose.offset_unv_mm.y;
+    sys.detector.pose.offset_unv_mm[2] = d.pose.offset_unv_mm.z;
+    sys.detector.pose.tilt_u_rad = d.pose.tilt_u_rad;
+    sys.detector.pose.tilt_v_rad = d.pose.tilt_v_rad;
+    sys.detector.pose.tilt_n_rad = d.pose.tilt_n_rad;
+    const SVolumeGridSpec& v = g.volume;
+    sys.volume.nx = v.nx;
+    sys.volume.ny = v.ny;
+    sys.volume.nz = v.nz;
+    sys.volume.voxel_x_mm = v.voxel_x_mm;
+    sys.volume.voxel_y_mm = v.voxel_y_mm;
+    sys.volume.voxel_z_mm = v.voxel_z_mm;
+    sys.volume.center_mm[0] = v.center_mm.x;
+    sys.volume.center_mm[1] = v.center_mm.y;
+    sys.volume.center_mm[2] = v.center_mm.z;
+    return sys;
+}
+
+// nominal views of the scan; false when the builder refuses it
+bool nominal_views(const yk_system& sys, std::vector<yk_view>& out)
+{
+    int n = 0;
+    const yk_status st = yk_geometry_build(&sys, nullptr, 0, &n);
+    if ((st != YK_OK && st != YK_ERR_INVALID_BUFFER) || n <= 0) return false;
+    out.resize(size_t(n));
+    return yk_geometry_build(&sys, out.data(), n, &n) == YK_OK;
+}
+
+yk_view to_view(const SConeProjGeomVec& f)
+{
+    yk_view v{};
+    const float4* in[5] = { &f.src, &f.detS, &f.detU, &f.detV, &f.angle };
+    float* out[5] = { v.src, v.detS, v.detU, v.detV, v.angle };
+    for (int k = 0; k < 5; ++k) {
+        out[k][0] = in[k]->x;
+        out[k][1] = in[k]->y;
+        out[k][2] = in[k]->z;
+        out[k][3] = in[k]->w;
+    }
+    return v;
+}
+
+// the frame checks of PreparedGeometry::initialize
+// (src/common/YkExecutionContext.hpp:22-36): finite vectors, non-degenerate
+// and orthogonal detector steps (|U.V| <= 1e-5 |U||V|), a finite angle
+bool usable_frame(const SConeProjGeomVec& f)
+{
+    for (const float4* a : { &f.src, &f.detS, &f.detU, &f.detV })
+        if (!std::isfinite(a->x) || !std::isfinite(a->y) || !std::isfinite(a->z)) return false;
+    const double uu = double(f.detU.x) * f.detU.x + double(f.detU.y) * f.detU.y + double(f.detU.z) * f.detU.z;
+    const double vv = double(f.detV.x) * f.detV.x + double(f.detV.y) * f.detV.y + double(f.detV.z) * f.detV.z;
+    const double uv = double(f.detU.x) * f.detV.x + double(f.detU.y) * f.detV.y + double(f.detU.z) * f.detV.z;
+    return uu > 0.0 && vv > 0.0 && std::abs(uv) <= 1e-5 * std::sqrt(uu * vv) && std::isfinite(f.angle.x);
+}
+
 // Scoped switch to the session's device, restoring the caller's.
 class DeviceScope {
 public:
@@ -103,6 +179,42 @@ public:
         if (spec.api_version != 2) return fail(EApiErrorCode::InvalidConfig, "unsupported SSystemSpec api_version");
         const SSystemConfig& g = spec.geometry;
         const SReconstructionSpec& rec = spec.reconstruction;
+        // flat-frame tail (YkReconstructionApi.cpp:146-167)
+        const bool frame_tail =
+            has_bytes(spec.struct_size, offsetof(SSystemSpec, flat_frame_count) + sizeof(spec.flat_frame_count));
+        if (spec.struct_size > offsetof(SSystemSpec, flat_frames) && !frame_tail)
+            return fail(EApiErrorCode::InvalidConfig, "truncated flat-frame tail");
+        const SConeProjGeomVec* frames = frame_tail ? spec.flat_frames : nullptr;
+        const int frame_count = frame_tail ? spec.flat_frame_count : 0;
+        if ((frames == nullptr) != (frame_count == 0) || frame_count < 0)
+            return fail(EApiErrorCode::InvalidConfig, "invalid flat-frame buffer / count");
+        if (frames) {
+            const EPipeline p = rec.pipeline;
+            if (g.detector != EDetectorKind::Flat ||
+                (g.trajectory != ETrajectoryKind::Circular && g.trajectory != ETrajectoryKind::Helical) ||
+                (p != EPipeline::ForwardProjection && p != EPipeline::SART && p != EPipeline::SIRT &&
+                 p != EPipeline::OSSART && p != EPipeline::CGLS && p != EPipeline::FDK && p != EPipeline::WFBP))
+                return fail(EApiErrorCode::UnsupportedCombination, "external flat frames are not supported by this pipeline");
+            if ((p == EPipeline::FDK || p == EPipeline::WFBP) && g.trajectory == ETrajectoryKind::Circular) {
+                // analytic reconstruction keeps its circular-orbit model: the
+                // frames must be the nominal trajectory within 1e-3
+                std::vector<yk_view> expected;
+                if (!nominal_views(c_system(g), expected) || int(expected.size()) != frame_count)
+                    return fail(EApiErrorCode::InvalidGeometry,
+                                "analytic reconstruction with external frames needs the matching nominal trajectory");
+                for (int i = 0; i < frame_count; ++i) {
+                    const yk_view a = expected[size_t(i)], b = to_view(frames[i]);
+                    const float* av[5] = { a.src, a.detS, a.detU, a.detV, a.angle };
+                    const float* bv[5] = { b.src, b.detS, b.detU, b.detV, b.angle };
+                    for (int k = 0; k < 5; ++k)
+                        for (int j = 0; j < 3; ++j)
+                            if (std::abs(av[k][j] - bv[k][j]) > 1e-3f)
+                                return fail(EApiErrorCode::UnsupportedCombination,
+                                            "external frames depart from the circular trajectory FDK assumes; "
+                                            "use an iterative reconstruction");
+                }
+            }
+        }
         const int det = int(g.detector), traj = int(g.trajectory), pipe = int(rec.pipeline);
         if (det < 0 || det > 1 || traj < 0 || traj > 3 || pipe < 0 || pipe > int(EPipeline::Statistical))
             return fail(EApiErrorCode::InvalidConfig, "pipeline, detector or trajectory enum out of range");
@@ -165,39 +277,10 @@ public:
                 return fail(EApiErrorCode::InvalidConfig, "FDK Parker mode or filter enum out of range");
         }
 
-        yk_system sys{};
+        const yk_system sys = c_system(g);
         const SRegularCircularScanSpec& c = g.circular;
-        sys.scan.total_views = c.total_views;
-        sys.scan.views_per_turn = c.views_per_turn;
-        sys.scan.start_angle_rad = c.start_angle_rad;
-        sys.scan.rotation_direction = c.rotation_direction;
-        sys.scan.sid_mm = c.sid_mm;
-        sys.scan.sdd_mm = c.sdd_mm;
-        sys.scan.z_mm = c.z_mm;
-        sys.scan.source_offset_mm[0] = c.source_offset_mm.x;
-        sys.scan.source_offset_mm[1] = c.source_offset_mm.y;
-        sys.scan.source_offset_mm[2] = c.source_offset_mm.z;
         const SFlatDetectorSpec& d = g.flat_detector;
-        sys.detector.channels = d.channels;
-        sys.detector.rows = d.rows;
-        sys.detector.channel_size_mm = d.channel_size_mm;
-        sys.detector.row_size_mm = d.row_size_mm;
-        sys.detector.pose.offset_unv_mm[0] = d.pose.offset_unv_mm.x;
-        sys.detector.pose.offset_unv_mm[1] = d.pose.offset_unv_mm.y;
-        sys.detector.pose.offset_unv_mm[2] = d.pose.offset_unv_mm.z;
-        sys.detector.pose.tilt_u_rad = d.pose.tilt_u_rad;
-        sys.detector.pose.tilt_v_rad = d.pose.tilt_v_rad;
-        sys.detector.pose.tilt_n_rad = d.pose.tilt_n_rad;
         const SVolumeGridSpec& v = g.volume;
-        sys.volume.nx = v.nx;
-        sys.volume.ny = v.ny;
-        sys.volume.nz = v.nz;
-        sys.volume.voxel_x_mm = v.voxel_x_mm;
-        sys.volume.voxel_y_mm = v.voxel_y_mm;
-        sys.volume.voxel_z_mm = v.voxel_z_mm;
-        sys.volume.center_mm[0] = v.center_mm.x;
-        sys.volume.center_mm[1] = v.center_mm.y;
-        sys.volume.center_mm[2] = v.center_mm.z;
         if (v.nx <= 0 || v.ny <= 0 || v.nz <= 0 || d.channels <= 0 || d.rows <= 0)
             return fail(EApiErrorCode::InvalidGeometry, "volume and detector sizes must be positive");
 
@@ -209,6 +292,19 @@ public:
         std::vector<yk_view> views(static_cast<size_t>(n));
         if (yk_status st = yk_geometry_build(&sys, views.data(), n, &n); st != YK_OK)
             return fail_status(st, "geometry");
+        if (frames) {
+            // PreparedGeometry::initialize with external frames
+            // (src/common/YkExecutionContext.hpp:22-36); its failure is the
+            // backend's, reported as InvalidGeometry
+            if (frame_count != c.total_views)
+                return fail(EApiErrorCode::InvalidGeometry, "one external frame per view of the scan is required");
+            for (int i = 0; i < frame_count; ++i)
+                if (!usable_frame(frames[i]))
+                    return fail(EApiErrorCode::InvalidGeometry,
+                                "external frame " + std::to_string(i) +
+                                    ": non-finite, degenerate or non-orthogonal detector vectors");
+            for (int i = 0; i < frame_count; ++i) views[size_t(i)] = to_view(frames[i]);
+        }
 
         device_ = spec.device;
         if (yk_status st = yk_ctx_create(device_, &ctx_); st != YK_OK) return fail_status(st, "device context");
@@ -225,7 +321,19 @@ public:
             desc.parker_mode = int(rec.fdk.parker.mode);
             desc.rotation_direction = c.rotation_direction;
             desc.storage = YK_F32;
-            if (yk_status st = yk_fdk_create_system(ctx_, &desc, &sys, &fdk_); st != YK_OK) {
+            yk_status st;
+            if (frames) {
+                // prepareWithGeometry (YkSession.cpp:131-163): the frames are
+                // the geometry; angles from angle.x, the direction from the
+                // first two, the range of the scan
+                desc.scan_range_rad = float(2.0 * 3.14159265358979323846 * c.total_views / c.views_per_turn);
+                desc.start_angle_rad = views[0].angle[0];
+                if (n >= 2) desc.rotation_direction = views[1].angle[0] >= views[0].angle[0] ? 1 : -1;
+                st = yk_fdk_create(ctx_, &desc, views.data(), n, d.channels, d.rows, &sys.volume, &fdk_);
+            } else {
+                st = yk_fdk_create_system(ctx_, &desc, &sys, &fdk_);
+            }
+            if (st != YK_OK) {
                 const bool r = fail_status(st, "FDK plan");
                 release_handles();
                 return r;
@@ -281,22 +389,22 @@ public:
         if (r.projection.element_count < view_elems_ * size_t(count) || r.volume.element_count < vol_elems_)
             return fail(EApiErrorCode::InvalidBuffer, "projection / volume element_count below what this call needs");
 
-        int done = views_, total = views_;
+        // progress (ScopedExecutionProgress, YkReconstructionApi.cpp:383-384):
+        // the iterative backends report each outer iteration; FDK and the
+        // forward projection report nothing
+        const bool tail = has_bytes(r.struct_size, offsetof(SExecutionRequest, progress_user_data) +
+                                                       sizeof(r.progress_user_data));
+        yk_progress_fn progress = tail ? r.progress_callback : nullptr;
+        void* progress_user = tail ? r.progress_user_data : nullptr;
         bool ok;
         if (pipeline_ == EPipeline::FDK)
             ok = execute_fdk(r, offset, count);
         else if (pipeline_ == EPipeline::ForwardProjection)
             ok = execute_fp(r);
         else
-            ok = execute_iterative(r, done, total);
+            ok = execute_iterative(r, progress, progress_user);
         if (!ok) return false;
-        if (pipeline_ == EPipeline::FDK) {
-            received_ += count;
-            done = received_;
-        }
-        const bool tail = has_bytes(r.struct_size, offsetof(SExecutionRequest, progress_user_data) +
-                                                       sizeof(r.progress_user_data));
-        if (tail && r.progress_callback) r.progress_callback(r.progress_user_data, done, total);
+        if (pipeline_ == EPipeline::FDK) received_ += count;
         return true;
     }
 
@@ -371,7 +479,7 @@ private:
     // OSSART the exact ones in golden-ratio subset order, CGLS the
     // robust-restart strategy; non_negative clamps at zero (not for CGLS).
     // iteration_count > 0 overrides the outer iterations of SIRT and OSSART.
-    bool execute_iterative(const SExecutionRequest& r, int& done, int& total)
+    bool execute_iterative(const SExecutionRequest& r, yk_progress_fn progress, void* progress_user)
     {
         if (r.volume.location != EMemoryLocation::Device ||
             (pipeline_ == EPipeline::CGLS && r.projection.location != EMemoryLocation::Device))
@@ -400,11 +508,10 @@ private:
             d.back_kind = back_kind_;
             d.check_interval = 1;
             d.patience = 1;
-            yk_iterative_stats st{};
-            st.struct_size = sizeof(st);
-            if (yk_status e = yk_cgls_ex(proj_, &d, d_b, r.volume.data, &st, nullptr); e != YK_OK)
+            d.progress_callback = progress;
+            d.progress_user_data = progress_user;
+            if (yk_status e = yk_cgls_ex(proj_, &d, d_b, r.volume.data, nullptr, nullptr); e != YK_OK)
                 return fail(EApiErrorCode::CudaError, std::string("CGLS: ") + yk_last_error());
-            done = st.completed_iterations;
         } else {
             yk_algebraic_desc d{};
             d.struct_size = sizeof(d);
@@ -423,11 +530,11 @@ private:
             d.tv_strength_reduction = 1.f;
             d.check_interval = 1;
             d.patience = 1;
+            d.progress_callback = progress;
+            d.progress_user_data = progress_user;
             if (yk_status e = yk_algebraic(proj_, &d, d_b, r.volume.data, nullptr); e != YK_OK)
                 return fail(EApiErrorCode::CudaError, std::string("algebraic reconstruction: ") + yk_last_error());
-            done = iterations;
         }
-        total = iterations;
         return true;
     }
 

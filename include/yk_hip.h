@@ -357,6 +357,9 @@ yk_status yk_cgls(yk_projector* projector, int forward_kind, int back_kind,
  * residual_norms (optional, iterations + 1 doubles): [0] = ||b - A x0||, then
  * per completed iteration the true (ROBUST) or recursive (ASTRA) residual,
  * repeated past an early stop. */
+/* progress callback of the iterative entry points: (user data, completed
+ * units, total units) */
+typedef void (*yk_progress_fn)(void* user_data, int completed, int total);
 typedef enum yk_cgls_strategy { YK_CGLS_ROBUST_RESTART = 0, YK_CGLS_ASTRA = 1 } yk_cgls_strategy;
 typedef struct yk_cgls_desc {
     uint32_t struct_size;            /* sizeof(yk_cgls_desc) */
@@ -377,6 +380,13 @@ typedef struct yk_cgls_desc {
     int minimum_iterations;          /* >= 0 */
     int check_interval;              /* >= 1 */
     int patience;                    /* >= 1 checks in a row */
+    /* optional tail, read when struct_size covers it (reportExecutionProgress,
+     * src/common/YkExecutionProgress.hpp:18-25; YkCglsBackends.hpp:147,402):
+     * called on the calling thread once the GPU work of each completed
+     * iteration has finished, with (completed, iterations); a rolled-back
+     * step reports nothing */
+    yk_progress_fn progress_callback;
+    void* progress_user_data;
 } yk_cgls_desc;
 /* IterativeConvergenceStatistics (src/Reconstruction/Iterative/Common/
  * YkIterativeConvergence.hpp:18-30) plus the CGLS stop reasons */
@@ -455,6 +465,13 @@ typedef struct yk_algebraic_desc {
     int check_interval;              /* >= 1 */
     int patience;                    /* >= 1 */
     int tv_dimensions;               /* 3 (Volume3D; 0 means 3) or 2 (Slice2D: no z differences) */
+    /* optional tail, read when struct_size covers it
+     * (AlgebraicReconstructorEx::run, YkAlgebraicReconstructorEx.hpp:164):
+     * called on the calling thread once the GPU work of each outer iteration
+     * (its subset updates, TV steps and bounds) has finished, with
+     * (completed, iterations) */
+    yk_progress_fn progress_callback;
+    void* progress_user_data;
 } yk_algebraic_desc;
 /* stats (optional) receives the completed outer iterations and the monitor's
  * last readings */

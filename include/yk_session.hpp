@@ -86,6 +86,18 @@ struct SForwardProjectionPose {
     float rotation_z_rad = 0.f;
 };
 
+// Per-view cone geometry (SConeProjGeomVec,
+// include/YKCBCT/geometry/YkProjectionGeometry.hpp:12-18), 80 bytes: source,
+// centre of detector pixel (0,0), one-pixel steps along U and V (mm);
+// angle.x is the acquisition angle (rad).
+struct SConeProjGeomVec {
+    float4 src;
+    float4 detS;
+    float4 detU;
+    float4 detV;
+    float4 angle;
+};
+
 struct SSystemConfig {
     EDetectorKind detector = EDetectorKind::Flat;
     ETrajectoryKind trajectory = ETrajectoryKind::Circular;
@@ -205,6 +217,13 @@ struct SSystemSpec {
     SSystemConfig geometry{};
     SReconstructionSpec reconstruction{};
     int device = 0;
+    // Optional flat-frame tail: calibrated per-view geometry, one frame per
+    // view of the scan, used in place of the builder's views by
+    // ForwardProjection, SART, SIRT, OSSART and CGLS.  FDK accepts frames
+    // only within 1e-3 of the nominal circular trajectory.  Host memory
+    // borrowed during initialize only (the session copies it).
+    const SConeProjGeomVec* flat_frames = nullptr;
+    int flat_frame_count = 0;
 };
 
 struct SExecutionRequest {
@@ -216,8 +235,9 @@ struct SExecutionRequest {
     int view_count = 0;  // 0: every prepared view
     bool clear_output = true;
     int iteration_count = 0;
-    // Optional tail, called on execute's thread after the GPU work of a
-    // successful call (completed, total = views received so far, views prepared).
+    // Optional tail: called on execute's thread once the GPU work of each
+    // outer iteration of SART / SIRT / OSSART / CGLS has finished
+    // (completed, total = iterations); FDK and ForwardProjection report nothing.
     void (*progress_callback)(void*, int completed, int total) = nullptr;
     void* progress_user_data = nullptr;
 };

@@ -44,14 +44,15 @@ def test_library_exports_the_session_factory():
 def test_session_facade_validation_without_device():
     """Session-API scenarios decided before any device work (struct_size,
     api_version, enum ranges, pose, unsupported combinations, scan validity,
-    execute before initialize): tests/session_check.cpp --host-only."""
+    execute before initialize, the flat-frame tail checks):
+    tests/session_check.cpp --host-only."""
     import subprocess
     from fdk_cuda_impl_amd.build import SESSION_CHECK
     assert os.path.exists(SESSION_CHECK), "build() compiles the Session check program"
     res = subprocess.run([SESSION_CHECK, "--host-only"], capture_output=True, text=True, timeout=60)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
-    assert res.stdout.count("PASS") == 7
+    assert res.stdout.count("PASS") == 8
 
 
 def test_struct_sizes_match_reference_pod_layout():

@@ -5,7 +5,12 @@ vs 13 + 19-view streaming, the streaming state machine and completion-time
 host output, reset / release / reuse, the forward-projection pose, the
 DLL Shepp-Logan FP -> FDK quality gates with 47-view streaming) and the
 iterative pipelines (SIRT smoke and reset / reuse, the SART / SIRT / OSSART /
-CGLS mapping onto the C-ABI solvers, buffer and request rules)."""
+CGLS mapping onto the C-ABI solvers, buffer and request rules), the
+flat-frame tail (external calibrated frames: tail checks, invariance of
+SART / SIRT / OSSART / CGLS / FDK under nominal frames, the forward
+projection of calibrated frames, an older SSystemSpec) and per-iteration
+progress (example/multispectrum_sim/tests/recon/backend/
+test_reconstruction_progress.cpp)."""
 import subprocess
 
 import pytest
@@ -20,4 +25,4 @@ def test_session_scenarios():
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "FAIL" not in res.stdout
-    assert res.stdout.count("PASS") == 21
+    assert res.stdout.count("PASS") == 26
