@@ -25,6 +25,9 @@ BP_JOSEPH_ADJOINT, BP_SIDDON_ADJOINT, BP_FDK, BP_FDK_PSEUDO_MATCHED, BP_JOSEPH_V
 PARKER_AUTO, PARKER_DISABLED, PARKER_ENABLED = 0, 1, 2
 FILTERS = {"none": 0, "ramlak": 1, "shepplogan": 2, "cosine": 3, "hann": 4, "hamming": 5,
            "blackman": 6, "butterworth": 7, "kaiser": 8, "tukey": 9}
+# YK_NOALLOC_MAX_ROWS: row factors (views x detector rows) one call of
+# yk_fdk_backproject_noalloc may use
+NOALLOC_MAX_ROWS = 4194304
 # yk_progress_fn: (user data, completed, total)
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int)
 STORAGE = {"f32": F32, "float32": F32, "f16": F16, "float16": F16, "bf16": BF16, "bfloat16": BF16}
@@ -184,6 +187,7 @@ SIGNATURES = {
     "yk_fdk_complete": (_I, [_P]),
     "yk_fdk_filter": (_I, [_P, _P, _I, _I, _P, _I, _P]),
     "yk_fdk_backproject": (_I, [_P, _P, _I, _P, _I, _I, _P, _I]),
+    "yk_fdk_backproject_noalloc": (_I, [_P, _P, _I, _P, _I, _I, _P, _I]),
     "yk_fdk_padded_n": (_I, [_P]),
     "yk_fdk_get_weights": (_I, [_P, _FP]),
     "yk_fdk_get_derived": (_I, [_P, _FP]),

@@ -363,7 +363,10 @@ class FdkPlan:
                _ptr(scales) if st != C.F32 else None)
         return out, (scales if st != C.F32 else None)
 
-    def backproject(self, filtered, scales=None, first: int = 0, volume=None, accumulate=False):
+    def backproject(self, filtered, scales=None, first: int = 0, volume=None, accumulate=False,
+                    no_alloc=False):
+        """Backprojection of filtered views [first, first + len(filtered)).
+        no_alloc: the allocation-free entry point (yk_fdk_backproject_noalloc)."""
         torch = _torch()
         st = {torch.float32: C.F32, torch.float16: C.F16, torch.bfloat16: C.BF16}[filtered.dtype]
         count = filtered.shape[0]
@@ -371,8 +374,8 @@ class FdkPlan:
             volume = torch.empty(self.volume_shape, dtype=torch.float32, device=filtered.device)
             accumulate = False
         _check_cuda(volume, torch.float32, self.volume_shape, "volume")
-        C.call("yk_fdk_backproject", self.handle, _ptr(filtered), st, _ptr(scales), first, count,
-               _ptr(volume), 1 if accumulate else 0)
+        C.call("yk_fdk_backproject_noalloc" if no_alloc else "yk_fdk_backproject", self.handle,
+               _ptr(filtered), st, _ptr(scales), first, count, _ptr(volume), 1 if accumulate else 0)
         return volume
 
     def execute(self, projection, volume, view_offset=0, clear_output=True):

@@ -105,6 +105,9 @@ yk_status yk_ctx_create(int device, yk_ctx** out)
         try {
             YK_HIP(hipSetDevice(device));
             YK_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+            // set-up of the allocation-free backprojection: the device
+            // address of this GPU's fixed scratch array
+            (void)bp_noalloc_scratch(device);
         } catch (...) {
             delete c;
             throw;
@@ -419,6 +422,17 @@ yk_status yk_fdk_backproject(yk_fdk* fdk, const void* d_filtered, int storage, c
         require(fdk != nullptr, YK_ERR_INVALID_CONFIG, "null FDK plan");
         fdk->ctx->activate();
         fdk->backproject(d_filtered, storage, d_row_scales, first, count, d_volume, accumulate);
+        fdk->ctx->finish();
+    });
+}
+
+yk_status yk_fdk_backproject_noalloc(yk_fdk* fdk, const void* d_filtered, int storage,
+    const float* d_row_scales, int first, int count, float* d_volume, int accumulate)
+{
+    return guard([&] {
+        require(fdk != nullptr, YK_ERR_INVALID_CONFIG, "null FDK plan");
+        fdk->ctx->activate();
+        fdk->backproject_noalloc(d_filtered, storage, d_row_scales, first, count, d_volume, accumulate);
         fdk->ctx->finish();
     });
 }

@@ -280,6 +280,49 @@ void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_sca
     launch_fdk_bp(p, s);
 }
 
+void yk_fdk::backproject_noalloc(const void* d_filtered, int storage, const float* d_scales, int first,
+    int count, float* d_volume, int accumulate)
+{
+    // the checks of backproject(), then the same kernels with every buffer
+    // beyond the inputs, the outputs and the plan's own tables in the
+    // library's fixed per-GPU scratch array
+    if (first < 0 || count <= 0 || first + count > n_views) fail(YK_ERR_INVALID_CONFIG, "view range outside the plan");
+    if (!d_filtered || !d_volume) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
+    storage_bytes(storage);
+    if (storage != YK_F32 && !d_scales) fail(YK_ERR_INVALID_BUFFER, "half / bfloat16 storage needs row scales");
+    const size_t quad = storage == YK_F32 ? 16 : 8;
+    if (size_t(lds_quads) * quad > 160 * 1024) fail(YK_ERR_UNSUPPORTED, "footprint exceeds LDS");
+    const size_t rows = size_t(count) * nv;
+    if (storage != YK_F32 && rows > kNoAllocRowFactors)
+        fail(YK_ERR_UNSUPPORTED, "view_count x detector rows exceeds YK_NOALLOC_MAX_ROWS (" +
+            std::to_string(kNoAllocRowFactors) + ") of the allocation-free backprojection");
+    unsigned* scratch = bp_noalloc_scratch(ctx->device);
+    int* gbits = reinterpret_cast<int*>(scratch);
+    unsigned* factors = scratch + kNoAllocHeader;
+    hipStream_t s = ctx->stream;
+    if (storage != YK_F32) launch_row_factors(d_scales, int(rows), storage, gbits, factors, s);
+    BpLaunch p{};
+    p.proj = d_filtered;
+    p.row_factors = storage == YK_F32 ? nullptr : factors;
+    p.gscale_bits = gbits;
+    p.coeffs = d_coeffs.get() + first;
+    p.tiles = d_tiles.get();
+    p.stage = d_stage.get();
+    p.tile_view_stride = n_views;
+    p.tile_view_offset = first;
+    p.volume = d_volume;
+    p.vol = vol;
+    p.nu = nu; p.nv = nv; p.count = count;
+    p.storage = storage;
+    p.accumulate = accumulate;
+    p.kahan = desc.kahan;
+    p.general = general ? 1 : 0;
+    p.nbx = nbx; p.nby = nby; p.nbz = nbz;
+    p.lds_quads = lds_quads;
+    p.bz = bz;
+    launch_fdk_bp(p, s);
+}
+
 void yk_fdk::execute(const float* projection, int ploc, size_t pcap, int offset, int count,
     float* volume, int vloc, size_t vcap, int clear_output)
 {

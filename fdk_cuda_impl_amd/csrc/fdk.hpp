@@ -14,6 +14,9 @@ struct yk_fdk {
         float* d_preweighted = nullptr);
     void backproject(const void* d_filtered, int storage, const float* d_scales, int first, int count,
         float* d_volume, int accumulate);
+    // backproject() with no device allocation (scratch in the fixed per-GPU array)
+    void backproject_noalloc(const void* d_filtered, int storage, const float* d_scales, int first, int count,
+        float* d_volume, int accumulate);
     void execute(const float* projection, int ploc, size_t pcap, int offset, int count, float* volume,
         int vloc, size_t vcap, int clear_output);
 

@@ -33,10 +33,41 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "yk_internal.hpp"
 
+// Fixed scratch of the allocation-free backprojection entry point
+// (yk_fdk_backproject_noalloc): word 0 receives the global row-scale bits,
+// words [kNoAllocHeader, kNoAllocHeader + kNoAllocRowFactors) the normalized
+// row factors of the views being backprojected.  One copy per GPU, held until
+// the process ends.  In host code `scratch` names a host-side stand-in, never
+// the device memory: kernels get the address bp_noalloc_scratch() looked up.
+static __device__ __align__(256) unsigned scratch[yk::kNoAllocHeader + yk::kNoAllocRowFactors];
+
 namespace yk {
+
+unsigned* bp_noalloc_scratch(int device)
+{
+    // looked up once per device, with that device current (the caller's
+    // hipSetDevice); each GPU holds its own copy of the array
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static unsigned* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) fail(YK_ERR_INVALID_CONFIG, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        YK_HIP(hipGetDevice(&cur));
+        if (cur != device) fail(YK_ERR_DEVICE, "scratch lookup with another device current");
+        void* p = nullptr;
+        YK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(YK_ERR_DEVICE, "no device address for the backprojection scratch");
+        addr[device] = static_cast<unsigned*>(p);
+    }
+    return addr[device];
+}
+
 namespace {
 
 // internal storage tag of the kernel: half texels already multiplied by their

@@ -242,6 +242,14 @@ int bp_block_z(int general, int storage, int kahan);
 // progress report of the iterative callers: waits for the stream's work, then
 // calls cb(user, completed, total) on this thread; no-op without a callback
 void report_progress(yk_progress_fn cb, void* user, int completed, int total, hipStream_t s);
+// Fixed per-GPU scratch of the allocation-free backprojection: a header of
+// kNoAllocHeader words (word 0: global row-scale bits) and room for
+// kNoAllocRowFactors row factors (views x detector rows of one call).
+constexpr size_t kNoAllocHeader = 64;
+constexpr size_t kNoAllocRowFactors = size_t(YK_NOALLOC_MAX_ROWS);
+// device address of that array on `device` (must be the current device),
+// looked up with hipGetSymbolAddress on the first call for the device
+unsigned* bp_noalloc_scratch(int device);
 
 struct ProjViewDev {
     float src[4];               // voxel-normalized, volume-centred

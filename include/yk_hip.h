@@ -315,6 +315,20 @@ yk_status yk_fdk_filter(yk_fdk* fdk, const float* d_projection, int first_view,
 yk_status yk_fdk_backproject(yk_fdk* fdk, const void* d_filtered, int storage,
     const float* d_row_scales, int first_view, int count, float* d_volume,
     int accumulate);
+/* yk_fdk_backproject without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Results
+ * equal yk_fdk_backproject's.  For half / bfloat16 storage the normalized row
+ * factors and the global row scale live in a fixed scratch array of the
+ * library, one per GPU (YK_NOALLOC_MAX_ROWS factors, 16 MiB, held until the
+ * process ends), and the kernel applies the factors while it stages texels
+ * (yk_fdk_backproject instead normalizes the rows into a workspace first).
+ * count * nv above YK_NOALLOC_MAX_ROWS is YK_ERR_UNSUPPORTED.  Calls that
+ * overlap on one GPU (other contexts or streams) race on that array; every
+ * call rewrites what it reads. */
+#define YK_NOALLOC_MAX_ROWS 4194304
+yk_status yk_fdk_backproject_noalloc(yk_fdk* fdk, const void* d_filtered, int storage,
+    const float* d_row_scales, int first_view, int count, float* d_volume, int accumulate);
 /* Introspection for tests: padded row length, weights (N/2+1 floats),
  * derived geometry (20 floats/view), coefficients (17 floats/view). */
 int yk_fdk_padded_n(const yk_fdk* fdk);

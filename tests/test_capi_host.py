@@ -149,3 +149,10 @@ def test_invalid_scan_rejected():
     with pytest.raises(yk.YkError) as e:
         yk.build_views(g)
     assert e.value.status == C.INVALID_GEOMETRY
+
+
+def test_noalloc_limit_matches_header():
+    """YK_NOALLOC_MAX_ROWS of include/yk_hip.h is the binding's constant."""
+    text = open(os.path.join(REPO, "include", "yk_hip.h")).read()
+    m = re.search(r"#define\s+YK_NOALLOC_MAX_ROWS\s+(\d+)", text)
+    assert m and int(m.group(1)) == C.NOALLOC_MAX_ROWS
