@@ -27,8 +27,10 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 METRIC = "FDK backprojection GUPS at 512³×720; FP16-stored vs FP32 at ≤1e-3 rel-RMSE"
-PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector peak (packed FP32 FMA)
-PEAK_FP32_NONPACKED_TFLOPS = 78.6  # one FP32 FMA per lane and cycle (this build emits no packed FP32)
+# MI355X FP32 vector peak; plain wave64 FP32 FMAs issue in 2 cycles on gfx950
+# and reach it, as do packed half FMAs / dot2 (4 cycles, two FMAs per lane)
+# (profiles/valu_instruction_costs_gfx950.txt)
+PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 FLOP_PER_UPDATE = {"f32": 25, "f16_kahan": 29}  # algorithmic FLOPs per voxel-view update
 
@@ -62,10 +64,12 @@ def measured_traffic():
 
 
 def measured_counters():
-    """VALU issue figures of the backprojection kernels at this workload from
-    the committed rocprofv3 SQ counter summary (SQ_INSTS_VALU x 4 cycles per
-    wave64 instruction / (1024 SIMDs x GRBM_GUI_ACTIVE per XCD)); None when
-    the summary is absent."""
+    """VALU figures of the backprojection kernels at this workload from the
+    committed rocprofv3 SQ counter summary: VALU instructions per update and
+    SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x GRBM_GUI_ACTIVE per XCD), the
+    busy fraction those counters imply when every VALU instruction takes 4
+    cycles (FP32 FMA / add / mul take 2 on gfx950); None when the summary is
+    absent."""
     path = os.path.join(REPO, "profiles", "fdk_bp_512cube_720views_counters.json")
     try:
         with open(path) as f:
@@ -244,15 +248,13 @@ def main():
     roofline = {
         "bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(),
-        "attainable_peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
-        "valu_busy": c16.get("valu_busy"),
         "valu_insts_per_update": c16.get("valu_insts_per_update"),
+        "sq_valu_quadcycle_frac": c16.get("valu_busy"),
         "hbm_frac": round(alg_bytes16 / (ms16 * 1e-3) / 1e9 / PEAK_HBM_GBS, 5),
-        "note": (f"VALU-issue-bound kernel (no MFMA); {FLOP_PER_UPDATE['f16_kahan']} algorithmic FLOP per "
-                 f"update over the packed FP32 peak; the half lerp and dot2 instructions do 4 FLOP each, so "
-                 f"the figure can exceed the {PEAK_FP32_NONPACKED_TFLOPS} TFLOP/s non-packed issue peak. "
-                 f"valu_busy (committed SQ counters) is the issue efficiency; HBM "
-                 f"{alg_bytes16 / (ms16 * 1e-3) / 1e9:.0f} GB/s algorithmic"),
+        "note": (f"VALU-bound kernel (no MFMA): {FLOP_PER_UPDATE['f16_kahan']} algorithmic FLOP per update "
+                 f"against the FP32 vector peak, which plain FP32 FMAs and packed half FMAs both reach on "
+                 f"gfx950; sq_valu_quadcycle_frac counts 4 cycles per VALU instruction (FP32 FMA / add / mul "
+                 f"take 2); HBM {alg_bytes16 / (ms16 * 1e-3) / 1e9:.0f} GB/s algorithmic"),
     }
     result = {
         "metric": METRIC, "value": round(gups16, 2), "unit": "GUPS", "n_gpus": 1,

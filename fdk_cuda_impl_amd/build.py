@@ -19,6 +19,7 @@ REPO = os.path.dirname(HERE)
 OBJ_DIR = os.path.join(REPO, "_build", "obj")
 LIB_PATH = os.path.join(HERE, "libyk_hip.so")
 SESSION_CHECK = os.path.join(HERE, "yk_session_check")
+VALU_COSTS = os.path.join(HERE, "yk_valu_costs")
 ARCH = os.environ.get("YK_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["capi.cpp", "geometry.cpp", "fdk.cpp", "projector.cpp", "algebraic.cpp", "cgls.cpp", "diagnostics.cpp", "session.cpp",
@@ -63,6 +64,7 @@ def build(jobs: int | None = None) -> str:
         raise RuntimeError(f"link failed\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB_PATH)
     build_session_check()
+    build_valu_costs()
     return LIB_PATH
 
 
@@ -79,6 +81,18 @@ def build_session_check() -> str:
         raise RuntimeError(f"session check build failed\n{res.stdout}\n{res.stderr}")
     os.replace(out + ".tmp", out)
     return out
+
+
+def build_valu_costs() -> str:
+    """Compile tests/valu_costs.hip, the VALU instruction-cost probe behind
+    profiles/valu_instruction_costs_gfx950.txt, next to the library."""
+    src = os.path.join(REPO, "tests", "valu_costs.hip")
+    cmd = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-x", "hip", src, "-o", VALU_COSTS + ".tmp"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"VALU cost probe build failed\n{res.stdout}\n{res.stderr}")
+    os.replace(VALU_COSTS + ".tmp", VALU_COSTS)
+    return VALU_COSTS
 
 
 if __name__ == "__main__":
