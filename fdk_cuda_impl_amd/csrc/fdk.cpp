@@ -84,6 +84,30 @@ void check_parker_coverage(const std::vector<yk_view>& views, int nu, int nv, do
         fail(YK_ERR_INVALID_CONFIG, "short-scan range below pi + 2 * fan angle");
 }
 
+// Geometry mode of the tilted-detector backprojection: 2 (a Newton step from
+// the previous slice's reciprocal of the depth denominator) when the
+// denominator changes by at most 2e-4 relative per slice anywhere in the
+// volume, so the step's relative error (the square of that change) stays
+// below FP32 resolution; else 1 (a reciprocal per sample).  The denominator
+// is affine in the voxel centre, so its minimum over the volume lies at a
+// corner.
+int general_mode(const std::vector<CoeffD>& coeffs, const VolumeGeom& g)
+{
+    double worst = 0.0;
+    for (const CoeffD& c : coeffs) {
+        double den_min = 1e300;
+        for (int a = 0; a < 8; ++a) {
+            const double X = g.ox + ((a & 1) ? (g.nx - 1) * double(g.vx) : 0.0);
+            const double Y = g.oy + ((a & 2) ? (g.ny - 1) * double(g.vy) : 0.0);
+            const double Z = g.oz + ((a & 4) ? (g.nz - 1) * double(g.vz) : 0.0);
+            den_min = std::min(den_min, c.cd[0] * X + c.cd[1] * Y + c.cd[2] * Z + c.cd[3]);
+        }
+        if (!(den_min > 0.0)) return 1;
+        worst = std::max(worst, std::fabs(c.cd[2]) * g.vz / den_min);
+    }
+    return worst <= 2e-4 ? 2 : 1;
+}
+
 }  // namespace
 }  // namespace yk
 
@@ -157,6 +181,7 @@ yk_fdk::yk_fdk(yk_ctx* c, const yk_fdk_desc& d, const yk_view* v, int n, int nu_
         if (b.cu[2] != 0.f || b.cd[2] != 0.f) general = true;
         if (b.cr[2] != 0.f) fail(YK_ERR_INVALID_GEOMETRY, "radial ray with a z component");
     }
+    geo_mode = general ? general_mode(coeffs, vol) : 0;
     d_fviews.upload(fv.data(), fv.size(), s);
     d_coeffs.upload(bc.data(), bc.size(), s);
     d_c64.upload(c64.data(), c64.size(), s);
@@ -273,7 +298,7 @@ void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_sca
     p.storage = kernel_storage;
     p.accumulate = accumulate;
     p.kahan = desc.kahan;
-    p.general = general ? 1 : 0;
+    p.general = geo_mode;
     p.nbx = nbx; p.nby = nby; p.nbz = nbz;
     p.lds_quads = lds_quads;
     p.bz = bz;
@@ -316,7 +341,7 @@ void yk_fdk::backproject_noalloc(const void* d_filtered, int storage, const floa
     p.storage = storage;
     p.accumulate = accumulate;
     p.kahan = desc.kahan;
-    p.general = general ? 1 : 0;
+    p.general = geo_mode;
     p.nbx = nbx; p.nby = nby; p.nbz = nbz;
     p.lds_quads = lds_quads;
     p.bz = bz;

@@ -28,6 +28,7 @@ struct yk_fdk {
     double scan_range = 0.0;
     bool parker = false;
     bool general = false;
+    int geo_mode = 0;              // 0 regular, 1 tilted (reciprocal per sample), 2 tilted (Newton steps)
     std::vector<yk_view> views;
     std::vector<yk::DerivedD> derived;
     std::vector<yk::CoeffD> coeffs;

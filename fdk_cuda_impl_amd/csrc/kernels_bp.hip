@@ -491,7 +491,15 @@ __device__ void stage_direct(void* lds, __amdgpu_buffer_rsrc_t data, __amdgpu_bu
 // ---------------------------------------------------------------------------
 constexpr int kBatch = 8;  // LDS reads issued before their results are consumed
 
-template <int STORAGE, int BZ, bool GENERAL>
+// Geometry modes of the kernel: the regular circular detector; any other
+// detector with one reciprocal per sample; any other detector whose depth
+// denominator changes by at most kNewtonStep (relative) per slice, where one
+// Newton step from the previous sample's reciprocal, r (2 - den r), is exact
+// to FP32 (relative error below the square of that change) and takes two
+// full-rate instructions instead of the quarter-rate reciprocal.
+constexpr int kRegular = 0, kGeneralRcp = 1, kGeneralNewton = 2;
+
+template <int STORAGE, int BZ, int GEO>
 __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& c, short4 t,
     float X, float Y, float Z0, float vz, float (&acc)[BZ])
 {
@@ -501,7 +509,7 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
     const float depth = c.cr[3] + c.cr[0] * X + c.cr[1] * Y;
     const float rd = __builtin_amdgcn_rcpf(depth);
     const float w = c.wbase * rd * rd;
-    if constexpr (!GENERAL) {
+    if constexpr (GEO == kRegular) {
         const float den = c.cd[3] + c.cd[0] * X + c.cd[1] * Y;
         const float r = __builtin_amdgcn_rcpf(den);
         const float u = (c.cu[3] + c.cu[0] * X + c.cu[1] * Y) * r - ul0;
@@ -571,30 +579,42 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
         // index mode (a move in and out per sample).  (Issuing the LDS reads of
         // eight samples before consuming them measured slower: the extra VGPRs
         // cost more than the exposed latency.)
+        float r = __builtin_amdgcn_rcpf(den);
+        // half u-weights times the distance weight: (w (1 - fu), w fu) =
+        // fu (-w, w) + (w, 0), so the dot product accumulates straight into acc
+        const half2_t wneg = { (_Float16)(-w), (_Float16)w };
+        const half2_t wlo = { (_Float16)w, (_Float16)0.f };
 #pragma unroll
         for (int k = 0; k < BZ; ++k) {
-            const float r = __builtin_amdgcn_rcpf(den);
+            if constexpr (GEO == kGeneralRcp) {
+                if (k > 0) r = __builtin_amdgcn_rcpf(den);
+            }
             // u, v >= 1 inside the footprint: truncation is floor
             const float u = un * r - ul0;
             const float v = vn * r - vl0;
             const float fu = __builtin_amdgcn_fractf(u), fv = __builtin_amdgcn_fractf(v);
             const char* qp = static_cast<const char*>(lds) + __mul24(int(v), rowb) + int(u) * qb;
-            float c0, c1;
             if constexpr (STORAGE == YK_F32) {
                 const float4 q = *reinterpret_cast<const float4*>(qp);
-                c0 = fmaf(fv, q.z, q.x);
-                c1 = fmaf(fv, q.w, q.y);
+                const float c0 = fmaf(fv, q.z, q.x);
+                const float c1 = fmaf(fv, q.w, q.y);
+                acc[k] = fmaf(w, fmaf(fu, c1 - c0, c0), acc[k]);
             } else {
+                // packed half lerps: (fv, fu) in one conversion, the v-lerp of
+                // both columns in one FMA, the weighted u-weights in one more,
+                // then a dot product accumulated in FP32
                 const uint2 q = *reinterpret_cast<const uint2*>(qp);
-                const half2_t h = __builtin_bit_cast(half2_t, q.x);
-                const half2_t d = __builtin_bit_cast(half2_t, q.y);
-                c0 = fmaf(fv, float(d.x), float(h.x));
-                c1 = fmaf(fv, float(d.y), float(h.y));
+                const half2_t vu = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(fv, fu));
+                const half2_t fvv = { vu.x, vu.x };
+                const half2_t fuu = { vu.y, vu.y };
+                const half2_t cc = __builtin_bit_cast(half2_t, q.y) * fvv + __builtin_bit_cast(half2_t, q.x);
+                const half2_t wu = fuu * wneg + wlo;
+                acc[k] = __builtin_amdgcn_fdot2(cc, wu, acc[k], false);
             }
-            acc[k] = fmaf(w, fmaf(fu, c1 - c0, c0), acc[k]);
             den += dd;
             un += du;
             vn += dvn;
+            if constexpr (GEO == kGeneralNewton) r = r * fmaf(-den, r, 2.f);
         }
     }
 }
@@ -607,7 +627,7 @@ constexpr int bp_min_blocks(int storage, int bz, bool kahan)
     return (storage != YK_F32 && kahan && bz == 32) ? 4 : 1;
 }
 
-template <int STORAGE, int BZ, bool GENERAL, bool KAHAN>
+template <int STORAGE, int BZ, int GEO, bool KAHAN>
 __global__ void __launch_bounds__(kThreads, bp_min_blocks(STORAGE, BZ, KAHAN))
 fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_factors,
     const int* __restrict__ gscale_bits, const BpCoeff* __restrict__ coeffs,
@@ -686,7 +706,7 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
                 prefetch_tile<STORAGE>(pf, m_next, data_rsrc(i + 1), fac_rsrc(i + 1), nu, t_next);
         }
         if (t_cur.z > 0 && active)
-            accumulate_view<STORAGE, BZ, GENERAL>(lds0 + cur, coeffs[i], t_cur, X, Y, Z0, vol.vz, acc);
+            accumulate_view<STORAGE, BZ, GEO>(lds0 + cur, coeffs[i], t_cur, X, Y, Z0, vol.vz, acc);
         if constexpr (KAHAN) {
             if ((i + 1) % kKahanViews == 0 || i + 1 == count) {
 #pragma unroll
@@ -724,12 +744,12 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
     }
 }
 
-template <int STORAGE, int BZ, bool GENERAL, bool KAHAN>
+template <int STORAGE, int BZ, int GEO, bool KAHAN>
 void launch_bz(const BpLaunch& p, const int* gbits, hipStream_t s)
 {
     dim3 grid(p.nbx, p.nby, p.nbz), block(kThreads);
     const size_t buf = size_t(p.lds_quads) * (STORAGE == YK_F32 ? 16 : 8);
-    auto k = fdk_bp_kernel<STORAGE, BZ, GENERAL, KAHAN>;
+    auto k = fdk_bp_kernel<STORAGE, BZ, GEO, KAHAN>;
     // double-buffer the tile when that costs no occupancy: workgroups per CU
     // allowed by the VGPRs (one wave per SIMD each) vs by 160 KB of LDS
     hipFuncAttributes attr{};
@@ -747,25 +767,30 @@ void launch_bz(const BpLaunch& p, const int* gbits, hipStream_t s)
         unsigned(buf), db);
 }
 
-template <int STORAGE, bool GENERAL, bool KAHAN>
+template <int STORAGE, int GEO, bool KAHAN>
 void launch_variant(const BpLaunch& p, const int* gbits, hipStream_t s, int bz)
 {
-    if (bz == 32) launch_bz<STORAGE, 32, GENERAL, KAHAN>(p, gbits, s);
-    else if (bz == 24) launch_bz<STORAGE, 24, GENERAL, KAHAN>(p, gbits, s);
-    else if (bz == 16) launch_bz<STORAGE, 16, GENERAL, KAHAN>(p, gbits, s);
-    else if (bz == 8) launch_bz<STORAGE, 8, GENERAL, KAHAN>(p, gbits, s);
+    if (bz == 32) launch_bz<STORAGE, 32, GEO, KAHAN>(p, gbits, s);
+    else if (bz == 16) launch_bz<STORAGE, 16, GEO, KAHAN>(p, gbits, s);
+    else if (bz == 8) launch_bz<STORAGE, 8, GEO, KAHAN>(p, gbits, s);
     else fail(YK_ERR_UNSUPPORTED, "unsupported block depth");
+}
+
+template <int STORAGE, int GEO>
+void launch_geo(const BpLaunch& p, const int* gbits, hipStream_t s, int bz)
+{
+    if (p.kahan) launch_variant<STORAGE, GEO, true>(p, gbits, s, bz);
+    else launch_variant<STORAGE, GEO, false>(p, gbits, s, bz);
 }
 
 template <int STORAGE>
 void launch_storage(const BpLaunch& p, const int* gbits, hipStream_t s, int bz)
 {
-    if (p.general) {
-        if (p.kahan) launch_variant<STORAGE, true, true>(p, gbits, s, bz);
-        else launch_variant<STORAGE, true, false>(p, gbits, s, bz);
-    } else {
-        if (p.kahan) launch_variant<STORAGE, false, true>(p, gbits, s, bz);
-        else launch_variant<STORAGE, false, false>(p, gbits, s, bz);
+    switch (p.general) {
+    case kRegular: launch_geo<STORAGE, kRegular>(p, gbits, s, bz); break;
+    case kGeneralRcp: launch_geo<STORAGE, kGeneralRcp>(p, gbits, s, bz); break;
+    case kGeneralNewton: launch_geo<STORAGE, kGeneralNewton>(p, gbits, s, bz); break;
+    default: fail(YK_ERR_INVALID_CONFIG, "unknown geometry mode");
     }
 }
 

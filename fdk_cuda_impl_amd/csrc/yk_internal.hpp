@@ -223,7 +223,8 @@ struct BpLaunch {
     int storage;
     int accumulate;
     int kahan;
-    int general;                // 1: u varies along z (tilted detector)
+    int general;                // 0 regular; 1 / 2 u varies along z (tilted detector):
+                                // a reciprocal per sample / Newton steps (fdk.cpp general_mode)
     int nbx, nby, nbz;
     int bz;                     // z slices per workgroup (32, 16 or 8)
     int lds_quads;              // LDS capacity in quads
