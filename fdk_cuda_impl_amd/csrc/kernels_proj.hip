@@ -188,30 +188,65 @@ joseph_fp_kernel(const float* __restrict__ vol, const float* __restrict__ vol_t,
     *dst = accumulate ? *dst + out : out;
 }
 
+// final store of one Joseph ray: the sum times the ray's step length, or
+// zero for a ray that misses the volume
+__device__ __forceinline__ void finish_joseph(float* proj, int i, int nu, int nv, int u, int v, bool pix,
+    bool hit, float acc, float L, int accumulate)
+{
+    if (!pix) return;
+    const float out = hit ? acc * L : 0.f;
+    float* dst = proj + ((size_t)i * nv + v) * nu + u;
+    *dst = accumulate ? *dst + out : out;
+}
+
+// Paired guarded volume copies of the Joseph projections.  A guarded copy
+// has kJosephGuard zero planes on both sides of both minor axes; its paired
+// form stores, at entry (slice c, row r, column e), the two guarded values
+// of rows r and r + 1 in column e.  One 16-byte load at (c, r, e) then
+// returns the four taps (r, e), (r + 1, e), (r, e + 1), (r + 1, e + 1) of a
+// bilinear sample: a wave64 load of 8 or 16 bytes per lane takes 16 cycles
+// of a CU's vector memory pipe on gfx950, so the one wider load halves the
+// cost of two 8-byte row loads.  Rows run 0 .. n_r + 2 guards - 2 (the last
+// row pairs with the trailing guard plane).
+constexpr int kJosephGuard = 2;
+typedef float float4u __attribute__((ext_vector_type(4), aligned(8)));
+
 // Joseph projection for separable views (main axis X or Y, detV along Z
 // only).  One wave per detector column (64 consecutive rays in v): a1, b1 and
 // hence f1 and its taps are the same for every lane, and the lanes' Z taps are
-// neighbouring entries of a Z-fastest volume copy ([x][y][z] for X-main,
-// [y][x][z] for Y-main views), so the gathers of a sample touch a few cache
-// lines per wave.  The copy carries kJosephGuard zero planes on both sides of
-// both minor axes: with the sample position clamped to [-2, n] both taps of
-// every sample lie in the copy, and a position outside (-1, n) reads only
-// zeros, so the kernel needs no per-sample range test (a slice outside a
-// lane's own range is outside (-1, n) on some minor axis).  A sample is two
-// 8-byte loads, (j1, j2..j2+1) and (j1 + 1, j2..j2+1), and three lerps.
-// Sample positions are those of joseph_fp_kernel (the same IEEE operations).
-constexpr int kJosephGuard = 2;
-typedef float float2u __attribute__((ext_vector_type(2), aligned(4)));
-
+// neighbouring entries of a paired Z-fastest copy (rows along y, columns
+// along z for X-main views; rows along x for Y-main views), so the gathers of
+// a sample touch a few cache lines per wave.  With the sample position
+// clamped to [-2, n] every tap lies in the copy, and a position outside
+// (-1, n) reads only zeros, so the kernel needs no per-sample range test (a
+// slice outside a lane's own range is outside (-1, n) on some minor axis).
+// A sample is one 16-byte load and three lerps; the first-minor weight and
+// tap row come from a per-wave LDS table built 64 slices at a time.  Sample
+// positions are those of joseph_fp_kernel (the same IEEE operations) and the
+// ray sum keeps its slice order.
+template <int CW>
 __global__ void __launch_bounds__(256)
 joseph_fp_sep_kernel(const float* __restrict__ vzx, const float* __restrict__ vzy,
     float* __restrict__ proj, const ProjViewDev* __restrict__ views, VolumeGeom g,
-    int nu, int nv, int accumulate)
+    int nu, int nv, int count, int accumulate)
 {
-    const int u = blockIdx.x * 4 + int(threadIdx.x >> 6);
-    const int v = blockIdx.y * 64 + int(threadIdx.x & 63);
-    const int i = blockIdx.z;
-    if (u >= nu) return;  // whole wave
+    // per wave, 64 slices at a time: the first-minor weight and the byte
+    // offset of its tap row (the same in every lane: a1, b1 are exact
+    // functions of the column u), built once per slice by one lane each and
+    // read back per sample by a broadcast LDS read
+    __shared__ uint2 tab[4][64];
+    const int wave = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
+    // The four waves take CW neighbouring detector columns of 4 / CW
+    // consecutive views: their rays nearly coincide, so the repeats of a tap
+    // row hit the CU's L1.  Column groups are remapped so that each XCD
+    // (workgroups go round-robin to the 8 XCDs; gridDim.x is a multiple of
+    // 8) sweeps a contiguous range of columns, whose neighbouring rays share
+    // tap rows in its L2.
+    const int group = int(blockIdx.x & 7u) * int(gridDim.x >> 3) + int(blockIdx.x >> 3);
+    const int u = group * CW + wave % CW;
+    const int v = blockIdx.y * 64 + lane;
+    const int i = int(blockIdx.z) * (4 / CW) + wave / CW;
+    if (u >= nu || i >= count) return;  // whole wave
     const bool pix = v < nv;
     const ProjViewDev pv = views[i];
     const AxisInfo ax = axis_info(pv.axis, g);
@@ -226,64 +261,91 @@ joseph_fp_sep_kernel(const float* __restrict__ vzx, const float* __restrict__ vz
     }
     wlo = __builtin_amdgcn_readfirstlane(wlo);
     whi = __builtin_amdgcn_readfirstlane(whi);
+    // lane 0 (v = 64 blockIdx.y < nv) holds the column's a1, b1
+    const float a1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.a1)));
+    const float b1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.b1)));
     const int p2 = ax.n2 + 2 * kJosephGuard;
-    const unsigned p2b = unsigned(p2) * 4u;
-    const size_t slice = size_t(ax.n1 + 2 * kJosephGuard) * p2;
+    const unsigned row_bytes = unsigned(p2) * 8u;
+    const size_t slice_bytes = size_t(ax.n1 + 2 * kJosephGuard - 1) * row_bytes;
     const float f1max = float(ax.n1), f2max = float(ax.n2);
     float acc = 0.f;
-    constexpr int kUnroll = 4;
-    for (int s0 = wlo; s0 <= whi; s0 += kUnroll) {
-        float2u ta[kUnroll], tb[kUnroll];
-        float w1[kUnroll], w2[kUnroll];
-#pragma unroll
-        for (int k = 0; k < kUnroll; ++k) {
-            const int s = min(s0 + k, whi);
-            float f1, f2;
-            joseph_sample(r, ax, s, f1, f2);
+    uint2* t = tab[wave];
+    for (int c = wlo; c <= whi; c += 64) {
+        {
+            const int st = min(c + lane, whi);
+            float f1 = joseph_minor(a1, b1, joseph_c0(st, ax.n0), ax.n1);
             f1 = __builtin_amdgcn_fmed3f(f1, -2.f, f1max);
-            f2 = __builtin_amdgcn_fmed3f(f2, -2.f, f2max);
-            w1[k] = __builtin_amdgcn_fractf(f1);
-            w2[k] = __builtin_amdgcn_fractf(f2);
-            const int j1 = int(floorf(f1)) + kJosephGuard, j2 = int(floorf(f2)) + kJosephGuard;
-            // slice base is wave-uniform (scalar), the in-slice byte offset a
-            // 32-bit lane value
-            const char* sl = reinterpret_cast<const char*>(base + size_t(s) * slice);
-            const unsigned o = unsigned(__mul24(j1, p2) + j2) * 4u;
-            const unsigned o2 = o + p2b;
-            ta[k] = *reinterpret_cast<const float2u*>(sl + o);
-            tb[k] = *reinterpret_cast<const float2u*>(sl + o2);
+            const int j1 = int(floorf(f1)) + kJosephGuard;
+            t[lane] = make_uint2(__float_as_uint(__builtin_amdgcn_fractf(f1)), unsigned(j1) * row_bytes);
         }
+        __builtin_amdgcn_wave_barrier();
+        const int cnt = min(64, whi - c + 1);
+        // the slice coordinate steps by exactly 1 (half-integers below 2^22);
+        // the slice base is wave-uniform
+        float c0 = joseph_c0(c, ax.n0);
+        const char* sl = reinterpret_cast<const char*>(base) + size_t(c) * slice_bytes;
+        int k = 0;
+        constexpr int kUnroll = 4;
+        for (; k + kUnroll <= cnt; k += kUnroll) {
+            float4u q[kUnroll];
+            float w1[kUnroll], w2[kUnroll];
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) {
-            const float a = fmaf(w2[k], ta[k].y - ta[k].x, ta[k].x);
-            const float b = fmaf(w2[k], tb[k].y - tb[k].x, tb[k].x);
-            const float c = fmaf(w1[k], b - a, a);
-            acc += s0 + k <= whi ? c : 0.f;  // the tail repeats slice whi
+            for (int j = 0; j < kUnroll; ++j) {
+                const uint2 e = t[k + j];
+                float f2 = joseph_minor(r.a2, r.b2, c0, ax.n2);
+                c0 += 1.f;
+                f2 = __builtin_amdgcn_fmed3f(f2, -2.f, f2max);
+                w1[j] = __uint_as_float(e.x);
+                w2[j] = __builtin_amdgcn_fractf(f2);
+                const unsigned o = e.y + unsigned(int(floorf(f2)) + kJosephGuard) * 8u;
+                q[j] = *reinterpret_cast<const float4u*>(sl + o);
+                sl += slice_bytes;
+            }
+#pragma unroll
+            for (int j = 0; j < kUnroll; ++j) {
+                const float a = fmaf(w2[j], q[j].z - q[j].x, q[j].x);
+                const float b = fmaf(w2[j], q[j].w - q[j].y, q[j].y);
+                acc += fmaf(w1[j], b - a, a);
+            }
         }
+        for (; k < cnt; ++k) {
+            const uint2 e = t[k];
+            float f2 = joseph_minor(r.a2, r.b2, c0, ax.n2);
+            c0 += 1.f;
+            f2 = __builtin_amdgcn_fmed3f(f2, -2.f, f2max);
+            const float w2 = __builtin_amdgcn_fractf(f2);
+            const unsigned o = e.y + unsigned(int(floorf(f2)) + kJosephGuard) * 8u;
+            const float4u q = *reinterpret_cast<const float4u*>(sl + o);
+            sl += slice_bytes;
+            const float a = fmaf(w2, q.z - q.x, q.x);
+            const float b = fmaf(w2, q.w - q.y, q.y);
+            acc += fmaf(__uint_as_float(e.x), b - a, a);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
-    if (!pix) return;
-    const float out = lo <= hi ? acc * r.L : 0.f;
-    float* dst = proj + ((size_t)i * nv + v) * nu + u;
-    *dst = accumulate ? *dst + out : out;
+    finish_joseph(proj, i, nu, nv, u, v, pix, lo <= hi, acc, r.L, accumulate);
 }
 
 // Joseph projection for any X- or Y-main view set (tilted detectors, lifted
-// sources), one lane per ray as joseph_fp_kernel, over guarded slab copies:
-// [x][z][y] for X-main and [y][z][x] for Y-main views, with kJosephGuard
-// zero planes around both minor axes.  A slice of the main axis is one
-// contiguous slab (scalar base, 32-bit lane offsets); the first minor axis is
-// the fastest, so the lanes of a wave (consecutive u) read neighbouring
-// entries.  As in joseph_fp_sep_kernel the sample position is clamped to
-// [-2, n] and positions outside (-1, n) read only zeros; a sample is two
-// 8-byte loads and three lerps.
+// sources), one lane per ray as joseph_fp_kernel, over paired slab copies:
+// rows along z and columns along the first minor axis (y for X-main, x for
+// Y-main views).  A slice of the main axis is one contiguous slab (scalar
+// base, 32-bit lane offsets); the lanes of a wave (consecutive u) read
+// neighbouring columns.  As in joseph_fp_sep_kernel the sample position is
+// clamped to [-2, n], positions outside (-1, n) read only zeros, and a
+// sample is one 16-byte load and three lerps.
 __global__ void __launch_bounds__(256)
 joseph_fp_slab_kernel(const float* __restrict__ gx, const float* __restrict__ gy,
     float* __restrict__ proj, const ProjViewDev* __restrict__ views, VolumeGeom g,
-    int nu, int nv, int accumulate)
+    int nu, int nv, int count, int accumulate)
 {
+    // the four waves take one 64-ray row segment of four consecutive views
+    // (nearly coincident rays: the repeats of a sample's cache lines hit the
+    // CU's L1)
     const int u = blockIdx.x * 64 + threadIdx.x;
-    const int v = blockIdx.y * 4 + threadIdx.y;
-    const int i = blockIdx.z;
+    const int v = blockIdx.y;
+    const int i = blockIdx.z * 4 + threadIdx.y;
+    if (i >= count) return;  // whole wave
     const bool pix = u < nu && v < nv;
     const ProjViewDev pv = views[i];
     const AxisInfo ax = axis_info(pv.axis, g);
@@ -299,73 +361,148 @@ joseph_fp_slab_kernel(const float* __restrict__ gx, const float* __restrict__ gy
     wlo = __builtin_amdgcn_readfirstlane(wlo);
     whi = __builtin_amdgcn_readfirstlane(whi);
     const int p1 = ax.n1 + 2 * kJosephGuard;
-    const unsigned p1b = unsigned(p1) * 4u;
-    const size_t slab = size_t(ax.n2 + 2 * kJosephGuard) * p1;
+    const size_t slab_bytes = size_t(ax.n2 + 2 * kJosephGuard - 1) * size_t(p1) * 8u;
     const float f1max = float(ax.n1), f2max = float(ax.n2);
     float acc = 0.f;
+    if (wlo > whi) return finish_joseph(proj, i, nu, nv, u, v, pix, lo <= hi, 0.f, r.L, accumulate);
+    // the slice coordinate steps by exactly 1 (half-integers below 2^22)
+    float c0 = joseph_c0(wlo, ax.n0);
+    const char* sl = reinterpret_cast<const char*>(base) + size_t(wlo) * slab_bytes;
     constexpr int kUnroll = 4;
-    for (int s0 = wlo; s0 <= whi; s0 += kUnroll) {
-        float2u ta[kUnroll], tb[kUnroll];
+    int s = wlo;
+    for (; s + kUnroll - 1 <= whi; s += kUnroll) {
+        float4u q[kUnroll];
         float w1[kUnroll], w2[kUnroll];
 #pragma unroll
         for (int k = 0; k < kUnroll; ++k) {
-            const int s = min(s0 + k, whi);
-            float f1, f2;
-            joseph_sample(r, ax, s, f1, f2);
+            float f1 = joseph_minor(r.a1, r.b1, c0, ax.n1);
+            float f2 = joseph_minor(r.a2, r.b2, c0, ax.n2);
+            c0 += 1.f;
             f1 = __builtin_amdgcn_fmed3f(f1, -2.f, f1max);
             f2 = __builtin_amdgcn_fmed3f(f2, -2.f, f2max);
             w1[k] = __builtin_amdgcn_fractf(f1);
             w2[k] = __builtin_amdgcn_fractf(f2);
             const int j1 = int(floorf(f1)) + kJosephGuard, j2 = int(floorf(f2)) + kJosephGuard;
-            const char* sl = reinterpret_cast<const char*>(base + size_t(s) * slab);
-            const unsigned o = unsigned(__mul24(j2, p1) + j1) * 4u;
-            const unsigned o2 = o + p1b;
-            ta[k] = *reinterpret_cast<const float2u*>(sl + o);
-            tb[k] = *reinterpret_cast<const float2u*>(sl + o2);
+            const unsigned o = unsigned(__mul24(j2, p1) + j1) * 8u;
+            q[k] = *reinterpret_cast<const float4u*>(sl + o);
+            sl += slab_bytes;
         }
 #pragma unroll
         for (int k = 0; k < kUnroll; ++k) {
-            const float a = fmaf(w1[k], ta[k].y - ta[k].x, ta[k].x);
-            const float b = fmaf(w1[k], tb[k].y - tb[k].x, tb[k].x);
-            const float c = fmaf(w2[k], b - a, a);
-            acc += s0 + k <= whi ? c : 0.f;
+            const float a = fmaf(w1[k], q[k].z - q[k].x, q[k].x);
+            const float b = fmaf(w1[k], q[k].w - q[k].y, q[k].y);
+            acc += fmaf(w2[k], b - a, a);
         }
     }
-    if (!pix) return;
-    const float out = lo <= hi ? acc * r.L : 0.f;
-    float* dst = proj + ((size_t)i * nv + v) * nu + u;
-    *dst = accumulate ? *dst + out : out;
+    for (; s <= whi; ++s) {
+        float f1 = joseph_minor(r.a1, r.b1, c0, ax.n1);
+        float f2 = joseph_minor(r.a2, r.b2, c0, ax.n2);
+        c0 += 1.f;
+        f1 = __builtin_amdgcn_fmed3f(f1, -2.f, f1max);
+        f2 = __builtin_amdgcn_fmed3f(f2, -2.f, f2max);
+        const float w1 = __builtin_amdgcn_fractf(f1), w2 = __builtin_amdgcn_fractf(f2);
+        const int j1 = int(floorf(f1)) + kJosephGuard, j2 = int(floorf(f2)) + kJosephGuard;
+        const unsigned o = unsigned(__mul24(j2, p1) + j1) * 8u;
+        const float4u q = *reinterpret_cast<const float4u*>(sl + o);
+        sl += slab_bytes;
+        const float a = fmaf(w1, q.z - q.x, q.x);
+        const float b = fmaf(w1, q.w - q.y, q.y);
+        acc += fmaf(w2, b - a, a);
+    }
+    finish_joseph(proj, i, nu, nv, u, v, pix, lo <= hi, acc, r.L, accumulate);
 }
 
-// [z][y][x] -> guarded [y][z][x] slab copy (x stays fastest: a row copy)
-__global__ void slab_copy_y_kernel(const float* __restrict__ in, float* __restrict__ out,
+// paired separable copy, X-main: [x][row y + guard][column z + guard];
+// blockIdx.z is the row, 32 x 32 (x, z) tiles of the two source rows
+// through LDS
+__global__ void pair_zfast_x_kernel(const float* __restrict__ in, float2* __restrict__ out,
+    int nx, int ny, int nz)
+{
+    __shared__ float tile[2][32][33];
+    const int r = blockIdx.z;
+    const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+    const size_t rows = size_t(ny) + 2 * kJosephGuard - 1, cols = size_t(nz) + 2 * kJosephGuard;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int z = z0 + k, x = x0 + threadIdx.x;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int y = r + h - kJosephGuard;
+            tile[h][k][threadIdx.x] = x < nx && z < nz && y >= 0 && y < ny ? in[(size_t(z) * ny + y) * nx + x] : 0.f;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int x = x0 + k, z = z0 + threadIdx.x;
+        if (x < nx && z < nz)
+            out[(size_t(x) * rows + r) * cols + z + kJosephGuard] = make_float2(tile[0][threadIdx.x][k], tile[1][threadIdx.x][k]);
+    }
+}
+
+// paired separable copy, Y-main: [y][row x + guard][column z + guard];
+// blockIdx.z is y, 32 x 32 (row, z) tiles (33 source columns of x) through
+// LDS
+__global__ void pair_zfast_y_kernel(const float* __restrict__ in, float2* __restrict__ out,
+    int nx, int ny, int nz)
+{
+    __shared__ float tile[32][34];
+    const int y = blockIdx.z;
+    const int r0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+    const int rows = nx + 2 * kJosephGuard - 1;
+    const size_t cols = size_t(nz) + 2 * kJosephGuard;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int z = z0 + k;
+        for (int t = threadIdx.x; t < 33; t += 32) {
+            const int x = r0 + t - kJosephGuard;
+            tile[k][t] = z < nz && x >= 0 && x < nx ? in[(size_t(z) * ny + y) * nx + x] : 0.f;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int r = r0 + k, z = z0 + threadIdx.x;
+        if (r < rows && z < nz)
+            out[(size_t(y) * rows + r) * cols + z + kJosephGuard] = make_float2(tile[threadIdx.x][k], tile[threadIdx.x][k + 1]);
+    }
+}
+
+// paired slab copy, X-main: [x][row z + guard][column y + guard];
+// blockIdx.z is the row, 32 x 32 (x, y) tiles of the two source planes
+// through LDS
+__global__ void pair_slab_x_kernel(const float* __restrict__ in, float2* __restrict__ out,
+    int nx, int ny, int nz)
+{
+    __shared__ float tile[2][32][33];
+    const int r = blockIdx.z;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const size_t rows = size_t(nz) + 2 * kJosephGuard - 1, cols = size_t(ny) + 2 * kJosephGuard;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int y = y0 + k, x = x0 + threadIdx.x;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int z = r + h - kJosephGuard;
+            tile[h][k][threadIdx.x] = x < nx && y < ny && z >= 0 && z < nz ? in[(size_t(z) * ny + y) * nx + x] : 0.f;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int x = x0 + k, y = y0 + threadIdx.x;
+        if (x < nx && y < ny)
+            out[(size_t(x) * rows + r) * cols + y + kJosephGuard] = make_float2(tile[0][threadIdx.x][k], tile[1][threadIdx.x][k]);
+    }
+}
+
+// paired slab copy, Y-main: [y][row z + guard][column x + guard] (x stays
+// fastest: row copies)
+__global__ void pair_slab_y_kernel(const float* __restrict__ in, float2* __restrict__ out,
     int nx, int ny, int nz)
 {
     const int x = blockIdx.x * 256 + threadIdx.x;
-    const int y = blockIdx.y, z = blockIdx.z;
+    const int y = blockIdx.y, r = blockIdx.z;
     if (x >= nx) return;
-    const size_t p1 = size_t(nx) + 2 * kJosephGuard, p2 = size_t(nz) + 2 * kJosephGuard;
-    out[(size_t(y) * p2 + z + kJosephGuard) * p1 + x + kJosephGuard] = in[(size_t(z) * ny + y) * nx + x];
-}
-
-// [z][y][x] -> guarded [x][z][y] slab copy: an x-y transpose per z plane,
-// 32 x 32 tiles through LDS
-__global__ void slab_copy_x_kernel(const float* __restrict__ in, float* __restrict__ out,
-    int nx, int ny, int nz)
-{
-    __shared__ float tile[32][33];
-    const int z = blockIdx.z;
-    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
-    for (int k = threadIdx.y; k < 32; k += 8) {
-        const int y = y0 + k, x = x0 + threadIdx.x;
-        if (x < nx && y < ny) tile[k][threadIdx.x] = in[(size_t(z) * ny + y) * nx + x];
-    }
-    __syncthreads();
-    const size_t p1 = size_t(ny) + 2 * kJosephGuard, p2 = size_t(nz) + 2 * kJosephGuard;
-    for (int k = threadIdx.y; k < 32; k += 8) {
-        const int x = x0 + k, y = y0 + threadIdx.x;
-        if (x < nx && y < ny) out[(size_t(x) * p2 + z + kJosephGuard) * p1 + y + kJosephGuard] = tile[threadIdx.x][k];
-    }
+    const size_t rows = size_t(nz) + 2 * kJosephGuard - 1, cols = size_t(nx) + 2 * kJosephGuard;
+    const int z = r - kJosephGuard;
+    const float a = z >= 0 && z < nz ? in[(size_t(z) * ny + y) * nx + x] : 0.f;
+    const float b = z + 1 >= 0 && z + 1 < nz ? in[(size_t(z + 1) * ny + y) * nx + x] : 0.f;
+    out[(size_t(y) * rows + r) * cols + x + kJosephGuard] = make_float2(a, b);
 }
 
 // [z][y][x] -> Z-fastest copy: out[x * sx + y * sy + z], one y row per
@@ -1205,56 +1342,51 @@ void launch_joseph_fp(const JosephLaunch& p, hipStream_t s)
 
 void launch_joseph_fp_sep(const JosephSepFpLaunch& p, hipStream_t s)
 {
-    dim3 block(256), grid((p.nu + 3) / 4, (p.nv + 63) / 64, p.count);
-    hipLaunchKernelGGL(joseph_fp_sep_kernel, grid, block, 0, s, p.vol_zx, p.vol_zy, p.proj, p.views, p.vol,
-        p.nu, p.nv, p.accumulate);
+    constexpr int kCols = 1;  // columns per workgroup (the other waves: further views)
+    const int groups = (p.nu + kCols - 1) / kCols;
+    dim3 block(256), grid((groups + 7) / 8 * 8, (p.nv + 63) / 64, (p.count + 4 / kCols - 1) / (4 / kCols));
+    hipLaunchKernelGGL(joseph_fp_sep_kernel<kCols>, grid, block, 0, s, p.vol_zx, p.vol_zy, p.proj, p.views,
+        p.vol, p.nu, p.nv, p.count, p.accumulate);
     YK_HIP(hipGetLastError());
 }
 
-void launch_transpose_zfast_guarded(const float* in, float* out, int nx, int ny, int nz, bool x_major,
+size_t joseph_pair_size(int nx, int ny, int nz, bool x_main, bool slab)
+{
+    const size_t n0 = size_t(x_main ? nx : ny), n1 = size_t(x_main ? ny : nx);
+    const size_t g = 2 * kJosephGuard;
+    // rows x columns of float2 per slice: (n_r + 2g - 1) x (n_e + 2g)
+    const size_t rows = (slab ? size_t(nz) : n1) + g - 1, cols = (slab ? n1 : size_t(nz)) + g;
+    return n0 * rows * cols * 2;
+}
+
+void launch_joseph_pair_copy(const float* in, float* out, int nx, int ny, int nz, bool x_main, bool slab,
     hipStream_t s)
 {
-    const size_t p2 = size_t(nz) + 2 * kJosephGuard;
-    const size_t p1 = size_t(x_major ? ny : nx) + 2 * kJosephGuard;
-    launch_fill(out, 0.f, joseph_guarded_size(nx, ny, nz, x_major), s);
-    dim3 block(32, 8), grid((nx + 31) / 32, (nz + 31) / 32, ny);
-    const size_t sx = x_major ? p1 * p2 : p2;
-    const size_t sy = x_major ? p2 : p1 * p2;
-    hipLaunchKernelGGL(transpose_zfast_kernel, grid, block, 0, s, in, out + kJosephGuard * p2 + kJosephGuard,
-        nx, ny, nz, sx, sy);
-    YK_HIP(hipGetLastError());
-}
-
-size_t joseph_guarded_size(int nx, int ny, int nz, bool x_major)
-{
-    return size_t(x_major ? nx : ny) * (size_t(x_major ? ny : nx) + 2 * kJosephGuard) *
-        (size_t(nz) + 2 * kJosephGuard);
-}
-
-size_t joseph_slab_size(int nx, int ny, int nz, bool x_main)
-{
-    return size_t(x_main ? nx : ny) * (size_t(nz) + 2 * kJosephGuard) *
-        (size_t(x_main ? ny : nx) + 2 * kJosephGuard);
-}
-
-void launch_slab_copy(const float* in, float* out, int nx, int ny, int nz, bool x_main, hipStream_t s)
-{
-    launch_fill(out, 0.f, joseph_slab_size(nx, ny, nz, x_main), s);
-    if (x_main) {
-        dim3 block(32, 8), grid((nx + 31) / 32, (ny + 31) / 32, nz);
-        hipLaunchKernelGGL(slab_copy_x_kernel, grid, block, 0, s, in, out, nx, ny, nz);
+    // the guard columns stay zero from the fill; every row is written
+    launch_fill(out, 0.f, joseph_pair_size(nx, ny, nz, x_main, slab), s);
+    float2* o = reinterpret_cast<float2*>(out);
+    const int g = 2 * kJosephGuard;
+    if (!slab && x_main) {
+        dim3 block(32, 8), grid((nx + 31) / 32, (nz + 31) / 32, ny + g - 1);
+        hipLaunchKernelGGL(pair_zfast_x_kernel, grid, block, 0, s, in, o, nx, ny, nz);
+    } else if (!slab) {
+        dim3 block(32, 8), grid((nx + g - 1 + 31) / 32, (nz + 31) / 32, ny);
+        hipLaunchKernelGGL(pair_zfast_y_kernel, grid, block, 0, s, in, o, nx, ny, nz);
+    } else if (x_main) {
+        dim3 block(32, 8), grid((nx + 31) / 32, (ny + 31) / 32, nz + g - 1);
+        hipLaunchKernelGGL(pair_slab_x_kernel, grid, block, 0, s, in, o, nx, ny, nz);
     } else {
-        dim3 block(256), grid((nx + 255) / 256, ny, nz);
-        hipLaunchKernelGGL(slab_copy_y_kernel, grid, block, 0, s, in, out, nx, ny, nz);
+        dim3 block(256), grid((nx + 255) / 256, ny, nz + g - 1);
+        hipLaunchKernelGGL(pair_slab_y_kernel, grid, block, 0, s, in, o, nx, ny, nz);
     }
     YK_HIP(hipGetLastError());
 }
 
 void launch_joseph_fp_slab(const JosephSepFpLaunch& p, hipStream_t s)
 {
-    dim3 block(64, 4), grid((p.nu + 63) / 64, (p.nv + 3) / 4, p.count);
+    dim3 block(64, 4), grid((p.nu + 63) / 64, p.nv, (p.count + 3) / 4);
     hipLaunchKernelGGL(joseph_fp_slab_kernel, grid, block, 0, s, p.vol_zx, p.vol_zy, p.proj, p.views, p.vol,
-        p.nu, p.nv, p.accumulate);
+        p.nu, p.nv, p.count, p.accumulate);
     YK_HIP(hipGetLastError());
 }
 

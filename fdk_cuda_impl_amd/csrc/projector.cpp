@@ -129,15 +129,15 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
         any_z = any_z || ax == 2;
     }
     if (kind == YK_FP_JOSEPH && all_sep) {
-        // guarded copies (the Siddon path reuses d_vol_zx unguarded; every
-        // call rewrites the copy it reads)
+        // paired guarded copies (the Siddon path reuses d_vol_zx as a plain
+        // copy; every call rewrites the copy it reads)
         if (any_x) {
-            grow(w.d_vol_zx, joseph_guarded_size(grid.nx, grid.ny, grid.nz, true));
-            launch_transpose_zfast_guarded(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
+            grow(w.d_vol_zx, joseph_pair_size(grid.nx, grid.ny, grid.nz, true, false));
+            launch_joseph_pair_copy(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, false, s);
         }
         if (any_y) {
-            grow(w.d_vol_zy, joseph_guarded_size(grid.nx, grid.ny, grid.nz, false));
-            launch_transpose_zfast_guarded(d_volume, w.d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, s);
+            grow(w.d_vol_zy, joseph_pair_size(grid.nx, grid.ny, grid.nz, false, false));
+            launch_joseph_pair_copy(d_volume, w.d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, false, s);
         }
         JosephSepFpLaunch p{};
         p.vol_zx = any_x ? w.d_vol_zx.get() : nullptr;
@@ -149,14 +149,14 @@ void yk_projector::project(int kind, const float* d_volume, float* d_proj, int f
         p.accumulate = accumulate;
         launch_joseph_fp_sep(p, s);
     } else if (kind == YK_FP_JOSEPH && !any_z) {
-        // X/Y-main views of any orientation: guarded slab copies
+        // X/Y-main views of any orientation: paired slab copies
         if (any_x) {
-            grow(w.d_vol_zx, joseph_slab_size(grid.nx, grid.ny, grid.nz, true));
-            launch_slab_copy(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, s);
+            grow(w.d_vol_zx, joseph_pair_size(grid.nx, grid.ny, grid.nz, true, true));
+            launch_joseph_pair_copy(d_volume, w.d_vol_zx.get(), grid.nx, grid.ny, grid.nz, true, true, s);
         }
         if (any_y) {
-            grow(w.d_vol_zy, joseph_slab_size(grid.nx, grid.ny, grid.nz, false));
-            launch_slab_copy(d_volume, w.d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, s);
+            grow(w.d_vol_zy, joseph_pair_size(grid.nx, grid.ny, grid.nz, false, true));
+            launch_joseph_pair_copy(d_volume, w.d_vol_zy.get(), grid.nx, grid.ny, grid.nz, false, true, s);
         }
         JosephSepFpLaunch p{};
         p.vol_zx = any_x ? w.d_vol_zx.get() : nullptr;

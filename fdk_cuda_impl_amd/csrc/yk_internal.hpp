@@ -280,10 +280,10 @@ struct JosephLaunch {
 void launch_joseph_fp(const JosephLaunch& p, hipStream_t s);
 void launch_transpose_xy(const float* in, float* out, int nx, int ny, int nz, hipStream_t s);
 
-// Joseph projection for separable views from Z-fastest volume copies
+// Joseph projection for separable views from paired Z-fastest volume copies
 struct JosephSepFpLaunch {
-    const float* vol_zx;        // guarded [x][y][z] copy (X-main views; may be null if none)
-    const float* vol_zy;        // guarded [y][x][z] copy (Y-main views; may be null if none)
+    const float* vol_zx;        // paired copy for X-main views (may be null if none)
+    const float* vol_zy;        // paired copy for Y-main views (may be null if none)
     float* proj;                // [count][nv][nu]
     const ProjViewDev* views;   // device, `count`
     VolumeGeom vol;
@@ -293,18 +293,18 @@ struct JosephSepFpLaunch {
 void launch_joseph_fp_sep(const JosephSepFpLaunch& p, hipStream_t s);
 // [z][y][x] -> [x][y][z] (x_major) or [y][x][z]
 void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz, bool x_major, hipStream_t s);
-// the same copy with two zero planes before and after each minor axis (y or
-// x, and z), the layout joseph_fp_sep_kernel reads; joseph_guarded_size
-// floats
-void launch_transpose_zfast_guarded(const float* in, float* out, int nx, int ny, int nz, bool x_major,
-    hipStream_t s);
-size_t joseph_guarded_size(int nx, int ny, int nz, bool x_major);
-// Joseph projection of X- or Y-main views of any orientation from guarded
-// slab copies ([x][z][y] in vol_zx, [y][z][x] in vol_zy; joseph_slab_size
-// floats each, made by launch_slab_copy)
+// Joseph projection of X- or Y-main views of any orientation from paired
+// slab copies (vol_zx / vol_zy made by launch_joseph_pair_copy with slab)
 void launch_joseph_fp_slab(const JosephSepFpLaunch& p, hipStream_t s);
-void launch_slab_copy(const float* in, float* out, int nx, int ny, int nz, bool x_main, hipStream_t s);
-size_t joseph_slab_size(int nx, int ny, int nz, bool x_main);
+// Paired guarded copy of a [z][y][x] volume for the Joseph projections:
+// float2 entries (value at row r, value at row r + 1) of a copy with two zero
+// planes before and after each minor axis.  Separable layout (slab false):
+// slices along the main axis (x for x_main, else y), rows along the other
+// of x and y, columns along z.  Slab layout: rows along z, columns along the
+// other of x and y.  joseph_pair_size floats.
+void launch_joseph_pair_copy(const float* in, float* out, int nx, int ny, int nz, bool x_main, bool slab,
+    hipStream_t s);
+size_t joseph_pair_size(int nx, int ny, int nz, bool x_main, bool slab);
 
 struct JosephAdjointLaunch {
     const float* proj;          // [count][nv][nu]
