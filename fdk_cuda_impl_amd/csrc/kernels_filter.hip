@@ -8,8 +8,10 @@
 // and both rows come out of one forward and one inverse transform.  The
 // transform is a mixed radix-8/4/2 Stockham FFT over two padded LDS ping-pong
 // buffers; twiddles come from an FP64-computed table, ordered pass by pass
-// (fft_twiddles), read from global memory.  Equivalent to the reference's
-// pad -> cuFFT R2C -> pointwise * w -> C2R -> crop chain
+// (fft_twiddles): each lane keeps those of its butterflies in registers
+// (spans of 64 and above) or reads them from a small LDS copy (span 8).
+// Equivalent to the reference's pad -> cuFFT R2C -> pointwise * w -> C2R ->
+// crop chain
 // (src/Reconstruction/Analytic/Circular/Flat/FDK/kernels/YkFDKFilterKernels.cu:32-140,
 // src/Filter/YkConvKernels.cu:109-220) with the preweight of
 // .../FDK/kernels/YkFDKPreWeightKernels.cu:289-405 applied on the way in.
@@ -31,9 +33,24 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// LDS index with one pad slot per 8 complex values: the radix-8 passes write
-// runs of 8 consecutive values per lane, which the padding spreads over banks
+// LDS index of the runtime-length transform: one pad slot per 8 complex
+// values (the radix-8 passes write runs of 8 consecutive values per lane,
+// which the padding spreads over banks)
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 3); }
+
+// LDS index of the compile-time transforms: C pad slots per 32 complex
+// values.  A half-wave's 32 lanes then read any 32 consecutive values of an
+// aligned block (every pass reads so) from distinct banks, and each pass
+// writes its output in the layout that spreads its own write pattern: C = 4
+// after the span-8 pass (runs of 8 values at a 64-value stride), C = 1 after
+// every other pass (runs of 8 single values at an 8-value stride, or
+// contiguous runs).  Both layouts fit in n + n/8 slots.
+template <int C>
+__device__ __forceinline__ int slot(int i) { return i + C * (i >> 5); }
+
+// pad slots of the layout a pass of span ns writes, and of the one it reads
+constexpr int fft_out_pad(int ns) { return ns == 8 ? 4 : 1; }
+constexpr int fft_in_pad(int ns) { return ns == 64 ? 4 : 1; }
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -126,26 +143,110 @@ __device__ float2* fft_forward(float2* a, float2* b, const float2* tw, int n)
 }
 
 // The same transform for a length known at compile time (n = NPT x the
-// workgroup size, every length of 256 and above): the pass sequence, spans
+// workgroup size, every length of 512 and above): the pass sequence, spans
 // and lane loops are constants, so the index arithmetic folds to shifts and
 // masks, and the first pass (span 1, all twiddles 1) multiplies by nothing.
-// The operations on the data are those of fft_forward.
+// Every lane keeps the twiddles of its butterflies (and the filter weights
+// of its last-pass bins) in registers for all the row pairs it filters: the
+// same entries of the pass-ordered table, so the operations on the data are
+// those of fft_forward.
+constexpr int fft_radix(int n, int ns) { return ns * 8 <= n ? 8 : (ns * 4 <= n ? 4 : 2); }
+
+// butterflies per lane of a pass of radix r
+constexpr int fft_iters(int n, int r) { return (n / r + kFilterThreads - 1) / kFilterThreads; }
+
+// The span-8 pass has 8 distinct twiddle sets (k = lane mod 8): it reads
+// them from a 56-entry LDS copy (8 distinct addresses per read: no bank
+// conflicts), which spares 14 registers.  Passes of span 64 and above keep
+// theirs in registers.
+constexpr int kTwLdsSpan = 8;
+constexpr bool fft_tw_in_regs(int ns) { return ns > kTwLdsSpan; }
+
+// registers of a pass's twiddles
+constexpr int fft_pass_tw_regs(int n, int ns)
+{
+    return fft_tw_in_regs(ns) ? fft_iters(n, fft_radix(n, ns)) * (fft_radix(n, ns) - 1) : 0;
+}
+
+// twiddle registers of the passes from span ns on
+constexpr int fft_tw_regs(int n, int ns)
+{
+    int total = 0;
+    while (ns < n) {
+        total += fft_pass_tw_regs(n, ns);
+        ns *= fft_radix(n, ns);
+    }
+    return total;
+}
+
+// span of the last pass
+constexpr int fft_last_ns(int n)
+{
+    int ns = 1;
+    while (ns * fft_radix(n, ns) < n) ns *= fft_radix(n, ns);
+    return ns;
+}
+
+// this lane's twiddles of the passes from span NS on (tw: that pass's block)
+template <int N, int NS>
+__device__ __forceinline__ void load_twiddles(float2* reg, const float2* __restrict__ tw)
+{
+    if constexpr (NS < N) {
+        constexpr int R = fft_radix(N, NS), M = N / R, IT = fft_iters(N, R);
+        if constexpr (fft_tw_in_regs(NS)) {
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int j = it * kFilterThreads + int(threadIdx.x);
+                const int k = j & (NS - 1);
+                const bool live = M % kFilterThreads == 0 || j < M;
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    reg[it * (R - 1) + r - 1] = live ? tw[(r - 1) * NS + k] : make_float2(0.f, 0.f);
+            }
+        }
+        load_twiddles<N, NS * R>(reg + fft_pass_tw_regs(N, NS), tw + (R - 1) * NS);
+    }
+}
+
+// this lane's filter weights of the last pass: bin idx + r NS of each of its
+// butterflies, w[b] for b <= N/2 and w[N - b] above (the response is even)
+template <int N>
+__device__ __forceinline__ void load_spectral_weights(float* reg, const float* __restrict__ weights)
+{
+    constexpr int NS = fft_last_ns(N), R = fft_radix(N, NS), M = N / R, IT = fft_iters(N, R);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int j = it * kFilterThreads + int(threadIdx.x);
+        const int k = j & (NS - 1);
+        const int idx = (j - k) * R + k;
+        const bool live = M % kFilterThreads == 0 || j < M;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int b = idx + r * NS;
+            reg[it * R + r] = live ? weights[b <= N / 2 ? b : N - b] : 0.f;
+        }
+    }
+}
+
 template <int R, int N, int NS, bool SPECTRAL = false>
-__device__ __forceinline__ void stockham_pass_ct(const float2* src, float2* dst, const float2* tw,
-    const float* __restrict__ weights = nullptr)
+__device__ __forceinline__ void stockham_pass_ct(const float2* src, float2* dst, const float2* twr,
+    const float* wr, const float2* tw_lds)
 {
     constexpr int m = N / R;
-#pragma unroll 1
-    for (int j0 = 0; j0 < m; j0 += kFilterThreads) {
-        const int j = j0 + int(threadIdx.x);
+#pragma unroll
+    for (int it = 0; it < fft_iters(N, R); ++it) {
+        const int j = it * kFilterThreads + int(threadIdx.x);
         if (m % kFilterThreads != 0 && j >= m) break;
         const int k = j & (NS - 1);
         float2 v[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = src[pidx(j + r * m)];
-        if constexpr (NS > 1) {
+        for (int r = 0; r < R; ++r) v[r] = src[slot<fft_in_pad(NS)>(j + r * m)];
+        if constexpr (fft_tw_in_regs(NS)) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[(r - 1) * NS + k]);
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twr[it * (R - 1) + r - 1]);
+        } else if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw_lds[(r - 1) * NS + k]);
         }
         dft<R>(v);
         const int idx = (j - k) * R + k;
@@ -153,80 +254,99 @@ __device__ __forceinline__ void stockham_pass_ct(const float2* src, float2* dst,
         for (int r = 0; r < R; ++r) {
             if constexpr (SPECTRAL) {
                 // last pass: bin idx + r NS in natural order; Y = conj(X W)
-                const int b = idx + r * NS;
-                const float w = weights[b <= N / 2 ? b : N - b];
+                const float w = wr[it * R + r];
                 v[r] = make_float2(v[r].x * w, -v[r].y * w);
             }
-            dst[pidx(idx + r * NS)] = v[r];
+            dst[slot<fft_out_pad(NS)>(idx + r * NS)] = v[r];
         }
     }
 }
 
-// SPECTRAL: the last pass also applies the filter weights and conjugates
+// SPECTRAL: the last pass also applies the filter weights and conjugates.
+// twr: the twiddle registers of the passes of span NS and after.
 template <int N, int NS, bool SPECTRAL>
-__device__ __forceinline__ float2* fft_forward_ct(float2* src, float2* dst, const float2* tw,
-    const float* __restrict__ weights)
+__device__ __forceinline__ float2* fft_forward_ct(float2* src, float2* dst, const float2* twr, const float* wr,
+    const float2* tw_lds)
 {
     if constexpr (NS >= N) {
-        (void)dst; (void)tw; (void)weights;
+        (void)dst; (void)twr; (void)wr; (void)tw_lds;
         return src;
     } else {
-        constexpr int R = NS * 8 <= N ? 8 : (NS * 4 <= N ? 4 : 2);
-        stockham_pass_ct<R, N, NS, SPECTRAL && NS * R >= N>(src, dst, tw, weights);
+        constexpr int R = fft_radix(N, NS);
+        stockham_pass_ct<R, N, NS, SPECTRAL && NS * R >= N>(src, dst, twr, wr, tw_lds);
         __syncthreads();
-        return fft_forward_ct<N, NS * R, SPECTRAL>(dst, src, tw + (R - 1) * NS, weights);
+        return fft_forward_ct<N, NS * R, SPECTRAL>(dst, src, twr + fft_pass_tw_regs(N, NS), wr, tw_lds);
     }
 }
 
-// NPT > 1 means n = NPT x the workgroup size exactly (the launcher picks
-// NPT = n / 256 for power-of-two n >= 512); NPT = 1 also serves n < 256
-template <int NPT>
-__device__ __forceinline__ float2* fft_rows(float2* a, float2* b, const float2* tw, int n)
-{
-    if constexpr (NPT > 1) {
-        (void)n;
-        return fft_forward_ct<NPT * kFilterThreads, 1, false>(a, b, tw, nullptr);
-    } else {
-        return fft_forward(a, b, tw, n);
-    }
-}
+// Cosine weight SDD_r / |q| * SDD_r / (2 SOD) of one detector row, q =
+// detS + u detU + v detV - src (.../FDK/kernels/YkFDKPreWeightKernels.cu:374,401-402),
+// times Parker: |q|^2 = |b|^2 + u (2 b.detU + u |detU|^2) with b the
+// row's u = 0 offset, a quadratic in u set up once per row.
+struct RowWeight {
+    float a, b, c, k;
+    const float* parker;
+};
 
-// cosine weight SDD_r / |q| * SDD_r / (2 SOD), q = detS + u detU + v detV - src
-// (.../FDK/kernels/YkFDKPreWeightKernels.cu:374,401-402), times Parker
-__device__ __forceinline__ float row_weight(const FilterView& fv, const float* __restrict__ parker_row,
-    int v, int u)
+__device__ __forceinline__ RowWeight row_weight_setup(const FilterView& fv, const float* __restrict__ parker_row,
+    int v)
 {
-    const float qx = (fv.detS[0] + fv.detV[0] * v - fv.src[0]) + fv.detU[0] * u;
-    const float qy = (fv.detS[1] + fv.detV[1] * v - fv.src[1]) + fv.detU[1] * u;
-    const float qz = (fv.detS[2] + fv.detV[2] * v - fv.src[2]) + fv.detU[2] * u;
-    const float r2 = fmaxf(qx * qx + qy * qy + qz * qz, 1e-20f);
-    float w = fv.sdd_radial * rsqrtf(r2) * fv.wgeom;
-    if (parker_row) w *= parker_row[u];
+    const float bx = fv.detS[0] + fv.detV[0] * v - fv.src[0];
+    const float by = fv.detS[1] + fv.detV[1] * v - fv.src[1];
+    const float bz = fv.detS[2] + fv.detV[2] * v - fv.src[2];
+    RowWeight w;
+    w.a = bx * bx + by * by + bz * bz;
+    w.b = 2.f * (bx * fv.detU[0] + by * fv.detU[1] + bz * fv.detU[2]);
+    w.c = fv.detU[0] * fv.detU[0] + fv.detU[1] * fv.detU[1] + fv.detU[2] * fv.detU[2];
+    w.k = fv.sdd_radial * fv.wgeom;
+    w.parker = parker_row;
     return w;
 }
 
-__device__ __forceinline__ float block_max(float v, float* red)
+__device__ __forceinline__ float row_weight(const RowWeight& w, int u)
 {
-    // wave64 reduction through LDS-free shuffles, then across waves
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) red[wave] = v;
-    __syncthreads();
-    float m = red[0];
-    for (int w = 1; w < kFilterThreads / 64; ++w) m = fmaxf(m, red[w]);
-    __syncthreads();
-    return m;
+    const float uf = float(u);
+    const float r2 = fmaxf(fmaf(uf, fmaf(uf, w.c, w.b), w.a), 1e-20f);
+    float x = w.k * rsqrtf(r2);
+    if (w.parker) x *= w.parker[u];
+    return x;
 }
 
-// power-of-two scale s with max|row| / s in (2^13, 2^14]; 1 for an all-zero row
-__device__ __forceinline__ float row_scale(float maxabs)
+// workgroup maxima of a and b (both rows of a pair at once)
+__device__ __forceinline__ void block_max2(float& a, float& b, float2* red)
 {
-    if (!(maxabs > 0.f) || !isfinite(maxabs)) return 1.f;
+    // wave64 reduction through shuffles, then across waves
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmaxf(a, __shfl_xor(a, off, 64));
+        b = fmaxf(b, __shfl_xor(b, off, 64));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[wave] = make_float2(a, b);
+    __syncthreads();
+    float2 m = red[0];
+    for (int w = 1; w < kFilterThreads / 64; ++w) m = make_float2(fmaxf(m.x, red[w].x), fmaxf(m.y, red[w].y));
+    __syncthreads();
+    a = m.x;
+    b = m.y;
+}
+
+// power-of-two scale s = 2^e with max|row| / s in (2^13, 2^14]; e = 0 for
+// an all-zero (or non-finite) row.  Returns e: s and 1 / s are exact powers
+// of two.
+__device__ __forceinline__ int row_scale_exp(float maxabs)
+{
+    if (!(maxabs > 0.f) || !isfinite(maxabs)) return 0;
     int e;
     const float m = frexpf(maxabs, &e);  // maxabs = m 2^e, m in [0.5, 1)
     (void)m;
-    return ldexpf(1.f, e - 14);
+    return e - 14;
 }
+
+// LDS index of the rows entering and leaving the transforms: the layout the
+// first pass reads and the last pass writes (pad 1 per 32 for compile-time
+// lengths; every pass sequence of n >= 512 ends on a span other than 8)
+template <int NPT>
+__device__ __forceinline__ int io_slot(int i) { return NPT > 1 ? slot<1>(i) : pidx(i); }
 
 template <int STORAGE>
 __device__ __forceinline__ void store_row(void* out, size_t base, int a, float val, float inv_scale)
@@ -255,9 +375,11 @@ __device__ __forceinline__ RowRef row_ref(int r, int nv)
 
 // Persistent workgroups: each filters row pairs p = blockIdx.x, + gridDim.x,
 // ... .  The raw texels of the next pair (NPT padded positions per lane and
-// row) are loaded into registers while the current pair is transformed.
+// row) are loaded into registers while the current pair is stored.  Lengths
+// of 512 and above run at most 128 registers per lane (4 workgroups per CU,
+// the LDS limit at n = 2048); 4096 at 2 workgroups per CU.
 template <int STORAGE, int NPT>
-__global__ void __launch_bounds__(kFilterThreads)
+__global__ void __launch_bounds__(kFilterThreads, NPT >= 16 ? 2 : 4)
 fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
     float* __restrict__ row_scales, const FilterView* __restrict__ views,
     const float* __restrict__ parker, const float* __restrict__ weights,
@@ -267,8 +389,20 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
     const int padded = pidx(n);
     float2* bufa = reinterpret_cast<float2*>(smem_raw);
     float2* bufb = bufa + padded;
-    const float2* tws = tw;  // pass-ordered table, read through the L1
-    __shared__ float red[kFilterThreads / 64];
+    __shared__ float2 red[kFilterThreads / 64];
+    // compile-time lengths: this lane's twiddles and last-pass filter
+    // weights, loaded once (the pass-ordered table and the weights)
+    constexpr int NC = NPT > 1 ? NPT * kFilterThreads : 2;
+    float2 twr[NPT > 1 ? fft_tw_regs(NC, 1) : 1];
+    float wr[NPT > 1 ? fft_iters(NC, fft_radix(NC, fft_last_ns(NC))) * fft_radix(NC, fft_last_ns(NC)) : 1];
+    __shared__ float2 tw_lds[7 * kTwLdsSpan];
+    if constexpr (NPT > 1) {
+        load_twiddles<NC, 1>(twr, tw);
+        load_spectral_weights<NC>(wr, weights);
+        // the span-8 block of the pass-ordered table (after the 7 entries of span 1)
+        if (threadIdx.x < 7 * kTwLdsSpan) tw_lds[threadIdx.x] = tw[7 + threadIdx.x];
+        __syncthreads();
+    }
 
     const int pairs = (rows + 1) / 2;
     float xa[NPT], xb[NPT];
@@ -294,8 +428,8 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
         const RowRef A = row_ref(ra, nv), B = row_ref(has_b ? rb : ra, nv);
         const FilterView fa = views[A.view];
         const FilterView fb = views[B.view];
-        const float* park_a = parker ? parker + (size_t)A.view * nu : nullptr;
-        const float* park_b = parker ? parker + (size_t)B.view * nu : nullptr;
+        const RowWeight wta = row_weight_setup(fa, parker ? parker + (size_t)A.view * nu : nullptr, A.v);
+        const RowWeight wtb = row_weight_setup(fb, parker ? parker + (size_t)B.view * nu : nullptr, B.v);
         // Y = conj(X W): the second forward transform then yields conj(IFFT(X W))
         float2* spec;
         if constexpr (NPT >= 8) {
@@ -309,8 +443,8 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
             for (int e = 0; e < NPT; ++e) {
                 const int m = int(threadIdx.x) + e * kFilterThreads;
                 const int ua = m - fa.start_u, ub = m - fb.start_u;
-                const float wa = (ua >= 0 && ua < nu) ? row_weight(fa, park_a, A.v, ua) : 0.f;
-                const float wb = (has_b && ub >= 0 && ub < nu) ? row_weight(fb, park_b, B.v, ub) : 0.f;
+                const float wa = (ua >= 0 && ua < nu) ? row_weight(wta, ua) : 0.f;
+                const float wb = (has_b && ub >= 0 && ub < nu) ? row_weight(wtb, ub) : 0.f;
                 wv[e] = make_float2(xa[e] * wa, xb[e] * wb);
             }
 #pragma unroll
@@ -321,11 +455,10 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
                 dft<8>(v);
                 const int j = t * kFilterThreads + int(threadIdx.x);
 #pragma unroll
-                for (int r = 0; r < 8; ++r) bufb[pidx(j * 8 + r)] = v[r];
+                for (int r = 0; r < 8; ++r) bufb[slot<fft_out_pad(1)>(j * 8 + r)] = v[r];
             }
             __syncthreads();
-            if (p + int(gridDim.x) < pairs) prefetch(p + gridDim.x);
-            spec = fft_forward_ct<N, 8, true>(bufb, bufa, tws + 7, weights);
+            spec = fft_forward_ct<N, 8, true>(bufb, bufa, twr, wr, tw_lds);
         } else {
             // cosine (and Parker) weights on the way into LDS
 #pragma unroll
@@ -333,17 +466,16 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
                 const int m = int(threadIdx.x) + e * kFilterThreads;
                 if (m < n) {
                     const int ua = m - fa.start_u, ub = m - fb.start_u;
-                    const float wa = (ua >= 0 && ua < nu) ? row_weight(fa, park_a, A.v, ua) : 0.f;
-                    const float wb = (has_b && ub >= 0 && ub < nu) ? row_weight(fb, park_b, B.v, ub) : 0.f;
-                    bufa[pidx(m)] = make_float2(xa[e] * wa, xb[e] * wb);
+                    const float wa = (ua >= 0 && ua < nu) ? row_weight(wta, ua) : 0.f;
+                    const float wb = (has_b && ub >= 0 && ub < nu) ? row_weight(wtb, ub) : 0.f;
+                    bufa[io_slot<NPT>(m)] = make_float2(xa[e] * wa, xb[e] * wb);
                 }
             }
             __syncthreads();
-            if (p + int(gridDim.x) < pairs) prefetch(p + gridDim.x);
             if constexpr (NPT > 1) {
-                spec = fft_forward_ct<NPT * kFilterThreads, 1, true>(bufa, bufb, tws, weights);
+                spec = fft_forward_ct<NC, 1, true>(bufa, bufb, twr, wr, tw_lds);
             } else {
-                spec = fft_forward(bufa, bufb, tws, n);
+                spec = fft_forward(bufa, bufb, tw, n);
                 for (int k = threadIdx.x; k < n; k += kFilterThreads) {
                     const float w = weights[k <= n / 2 ? k : n - k];
                     const float2 x = spec[pidx(k)];
@@ -353,7 +485,9 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
             }
         }
         float2* other = spec == bufa ? bufb : bufa;
-        float2* res = fft_rows<NPT>(spec, other, tws, n);
+        float2* res;
+        if constexpr (NPT > 1) res = fft_forward_ct<NC, 1, false>(spec, other, twr, nullptr, tw_lds);
+        else res = fft_forward(spec, other, tw, n);
 
         // crop: row a = Re(res), row b = -Im(res)
         float inv_a = 1.f, inv_b = 1.f;
@@ -361,26 +495,30 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
             float ma = 0.f, mb = 0.f;
             for (int a = threadIdx.x; a < nu; a += kFilterThreads) {
                 const int ja = fa.start_u + a, jb = fb.start_u + a;
-                if (ja >= 0 && ja < n) ma = fmaxf(ma, fabsf(res[pidx(ja)].x));
-                if (jb >= 0 && jb < n) mb = fmaxf(mb, fabsf(res[pidx(jb)].y));
+                if (ja >= 0 && ja < n) ma = fmaxf(ma, fabsf(res[io_slot<NPT>(ja)].x));
+                if (jb >= 0 && jb < n) mb = fmaxf(mb, fabsf(res[io_slot<NPT>(jb)].y));
             }
-            ma = block_max(ma, red);
-            mb = block_max(mb, red);
-            const float sa = row_scale(ma), sb = row_scale(mb);
-            inv_a = 1.f / sa;
-            inv_b = 1.f / sb;
+            block_max2(ma, mb, red);
+            const int ea = row_scale_exp(ma), eb = row_scale_exp(mb);
+            const float sa = ldexpf(1.f, ea), sb = ldexpf(1.f, eb);
+            inv_a = ldexpf(1.f, -ea);
+            inv_b = ldexpf(1.f, -eb);
             if (threadIdx.x == 0) {
                 row_scales[ra] = sa;
                 if (has_b) row_scales[rb] = sb;
             }
         }
+        // the next pair's texels load while this pair is stored (issued
+        // here, not before the transforms, so that they hold no registers
+        // through the passes and the row maxima)
+        if (p + int(gridDim.x) < pairs) prefetch(p + gridDim.x);
         const size_t base_a = (size_t)ra * nu, base_b = (size_t)rb * nu;
         for (int a = threadIdx.x; a < nu; a += kFilterThreads) {
             const int ja = fa.start_u + a;
-            store_row<STORAGE>(out, base_a, a, (ja >= 0 && ja < n) ? res[pidx(ja)].x : 0.f, inv_a);
+            store_row<STORAGE>(out, base_a, a, (ja >= 0 && ja < n) ? res[io_slot<NPT>(ja)].x : 0.f, inv_a);
             if (has_b) {
                 const int jb = fb.start_u + a;
-                store_row<STORAGE>(out, base_b, a, (jb >= 0 && jb < n) ? -res[pidx(jb)].y : 0.f, inv_b);
+                store_row<STORAGE>(out, base_b, a, (jb >= 0 && jb < n) ? -res[io_slot<NPT>(jb)].y : 0.f, inv_b);
             }
         }
         // the next pair overwrites bufa, which the crop (res) may be
@@ -427,7 +565,7 @@ preweight_kernel(const float* __restrict__ proj, float* __restrict__ out, const 
     const int v = blockIdx.y, i = blockIdx.z;
     if (u >= nu) return;
     const size_t at = ((size_t)i * nv + v) * nu + u;
-    out[at] = proj[at] * row_weight(views[i], parker ? parker + (size_t)i * nu : nullptr, v, u);
+    out[at] = proj[at] * row_weight(row_weight_setup(views[i], parker ? parker + (size_t)i * nu : nullptr, v), u);
 }
 
 }  // namespace
@@ -443,7 +581,7 @@ void launch_preweight(const FilterLaunch& p, float* out, hipStream_t s)
 void launch_fdk_filter(const FilterLaunch& p, hipStream_t s)
 {
     const int rows = p.count * p.nv;
-    // two padded ping-pong buffers (the twiddles stay in global memory: an
+    // two padded ping-pong buffers (the twiddles live in registers: a full
     // LDS copy would cost a workgroup per CU)
     const size_t lds = (size_t(2) * (p.n + p.n / 8)) * sizeof(float2);
     if (lds > 160 * 1024) fail(YK_ERR_UNSUPPORTED, "padded row too long for the LDS filter");
