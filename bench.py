@@ -210,6 +210,7 @@ def main():
     v32 = torch.empty(plan32.volume_shape, device="cuda")
     v16 = torch.empty(plan16.volume_shape, device="cuda")
     filt_ms = time_launches(lambda: plan16.filter(proj, out=f16, scales=s16), 3, 1, stream)
+    filt32_ms = time_launches(lambda: plan32.filter(proj, out=f32), 3, 1, stream)
     ms32 = time_launches(lambda: plan32.backproject(f32, None, volume=v32), args.steps, args.warmup, stream)
     print(f"[bench] fp32 backprojection {ms32:.2f} ms", file=sys.stderr, flush=True)
     ms16 = time_launches(lambda: plan16.backproject(f16, s16, volume=v16), args.steps, args.warmup, stream)
@@ -230,6 +231,29 @@ def main():
         del tf, ts
     del vt
     print(f"[bench] general path {general}", file=sys.stderr, flush=True)
+    # Host -> device end to end: the whole FDK (filter + backprojection of all
+    # views) from pageable host projections into a device volume through
+    # yk_fdk_execute, the path Session::Execute takes (pinned staging, H2D on a
+    # copy stream overlapping the GPU work).  Best of two runs per storage.
+    proj_cpu = proj.cpu().numpy()
+    host_e2e = {"workload": "as above, projections in pageable host memory (%.2f GB FP32), volume on the device"
+                            % (proj_cpu.nbytes / 1e9)}
+    vh = torch.empty(plan32.volume_shape, device="cuda")
+    for storage, kahan in (("f32", False), ("f16", True)):
+        hplan = yk.FdkPlan.from_geometry(ctx, geom, filter="hann", storage=storage, kahan=kahan)
+        best = float("inf")
+        for _ in range(2):
+            hplan.reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            hplan.execute(proj_cpu, vh, view_offset=0, clear_output=True)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        hplan.close()
+        host_e2e[f"{storage}{'_kahan' if kahan else ''}_ms"] = round(best * 1e3, 2)
+    host_e2e["pcie_gb_per_s"] = round(proj_cpu.nbytes / (host_e2e["f32_ms"] * 1e-3) / 1e9, 1)
+    del vh
+    print(f"[bench] host-to-device end to end {host_e2e}", file=sys.stderr, flush=True)
     proj_leg = None
     if not args.no_projectors:
         proj_leg = projector_leg(yk, phantoms, ctx, stream)
@@ -271,13 +295,15 @@ def main():
         "fp32_gups": round(gups32, 2), "fp32_ms_per_step": round(ms32, 3),
         "fp16_speedup_vs_fp32": round(ms32 / ms16, 3),
         "filter_ms_fp16": round(filt_ms, 3),
+        "filter_ms_fp32": round(filt32_ms, 3),
+        "fdk_host_to_device_end_to_end": host_e2e,
         "joseph_fp_s": round(fp_s, 3),
         "projectors": proj_leg,
         "general_path_backprojection": general,
     }
     if not args.no_cpu:
         print("[bench] FP64 oracle check", file=sys.stderr, flush=True)
-        acc, cpu = oracle_check(views, proj.cpu().numpy(), geom,
+        acc, cpu = oracle_check(views, proj_cpu, geom,
                                 {"f16": v16.cpu().numpy(), "f32": v32.cpu().numpy()}, args)
         result["accuracy_vs_fp64"] = acc
         result["cpu_baseline"] = cpu

@@ -5,6 +5,8 @@
 #include <cstdint>
 #include <cmath>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "fdk.hpp"
 
@@ -348,6 +350,114 @@ void yk_fdk::backproject_noalloc(const void* d_filtered, int storage, const floa
     launch_fdk_bp(p, s);
 }
 
+namespace {
+
+// memcpy with up to 8 host threads (pageable <-> pinned staging runs at
+// several times the one-thread rate); the threads join before it returns
+void copy_parallel(void* dst, const void* src, size_t bytes)
+{
+    const size_t min_part = size_t(8) << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(8, hw), std::max<size_t>(1, bytes / min_part));
+    char* d = static_cast<char*>(dst);
+    const char* sp = static_cast<const char*>(src);
+    if (nt <= 1) {
+        std::memcpy(d, sp, bytes);
+        return;
+    }
+    const size_t part = (bytes / nt + 4095) & ~size_t(4095);
+    std::vector<std::thread> threads;
+    size_t done = part;  // the calling thread copies [0, part)
+    for (size_t off = part; off < bytes; off += part) {
+        const size_t len = std::min(part, bytes - off);
+        try {
+            threads.emplace_back([d, sp, off, len] { std::memcpy(d + off, sp + off, len); });
+            done = off + len;
+        } catch (...) {
+            break;  // no thread available: the rest is copied here
+        }
+    }
+    std::memcpy(d, sp, std::min(part, bytes));
+    if (done < bytes) std::memcpy(d + done, sp + done, bytes - done);
+    for (auto& t : threads) t.join();
+}
+
+}  // namespace
+
+yk_fdk::~yk_fdk()
+{
+    for (int k = 0; k < 2; ++k) {
+        if (ev_pinned[k]) (void)hipEventDestroy(ev_pinned[k]);
+        if (ev_consumed[k]) (void)hipEventDestroy(ev_consumed[k]);
+    }
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+}
+
+int yk_fdk::stage_chunk_views() const
+{
+    // about 128 MiB of FP32 projections per staged chunk (two pinned slots)
+    const size_t view_bytes = size_t(nu) * nv * 4;
+    const size_t target = size_t(128) << 20;
+    return int(std::max<size_t>(1, std::min<size_t>(size_t(batch), target / view_bytes)));
+}
+
+void yk_fdk::ensure_staging(size_t slot_bytes)
+{
+    for (int k = 0; k < 2; ++k) {
+        pinned[k].ensure(slot_bytes);
+        if (!ev_pinned[k]) YK_HIP(hipEventCreateWithFlags(&ev_pinned[k], hipEventDisableTiming));
+        if (!ev_consumed[k]) YK_HIP(hipEventCreateWithFlags(&ev_consumed[k], hipEventDisableTiming));
+    }
+    if (!copy_stream) YK_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+}
+
+void yk_fdk::upload_host(float* d_dst, const float* host, size_t bytes)
+{
+    // host -> pinned slot -> device on the plan's stream, two slots in flight
+    const size_t slot = size_t(stage_chunk_views()) * nu * nv * 4;
+    ensure_staging(slot);
+    hipStream_t s = ctx->stream;
+    const char* src = reinterpret_cast<const char*>(host);
+    char* dst = reinterpret_cast<char*>(d_dst);
+    for (size_t off = 0, i = 0; off < bytes; off += slot, ++i) {
+        const int k = int(i & 1);
+        const size_t len = std::min(slot, bytes - off);
+        if (pinned_used[k]) YK_HIP(hipEventSynchronize(ev_pinned[k]));
+        copy_parallel(pinned[k].get(), src + off, len);
+        YK_HIP(hipMemcpyAsync(dst + off, pinned[k].get(), len, hipMemcpyHostToDevice, s));
+        YK_HIP(hipEventRecord(ev_pinned[k], s));
+        pinned_used[k] = true;
+    }
+}
+
+void yk_fdk::download_host(float* host, const float* d_src, size_t bytes)
+{
+    // device -> pinned slot (plan's stream) -> host, the next slot's copy in
+    // flight while the host drains the current one
+    const size_t slot = size_t(stage_chunk_views()) * nu * nv * 4;
+    ensure_staging(slot);
+    hipStream_t s = ctx->stream;
+    const char* src = reinterpret_cast<const char*>(d_src);
+    char* dst = reinterpret_cast<char*>(host);
+    const size_t n = (bytes + slot - 1) / slot;
+    auto issue = [&](size_t i) {
+        const int k = int(i & 1);
+        const size_t off = i * slot, len = std::min(slot, bytes - off);
+        if (pinned_used[k]) YK_HIP(hipEventSynchronize(ev_pinned[k]));
+        YK_HIP(hipMemcpyAsync(pinned[k].get(), src + off, len, hipMemcpyDeviceToHost, s));
+        YK_HIP(hipEventRecord(ev_pinned[k], s));
+        pinned_used[k] = true;
+    };
+    for (size_t i = 0; i < n && i < 2; ++i) issue(i);
+    for (size_t i = 0; i < n; ++i) {
+        const int k = int(i & 1);
+        const size_t off = i * slot, len = std::min(slot, bytes - off);
+        YK_HIP(hipEventSynchronize(ev_pinned[k]));
+        copy_parallel(dst + off, pinned[k].get(), len);
+        if (i + 2 < n) issue(i + 2);
+    }
+}
+
 void yk_fdk::execute(const float* projection, int ploc, size_t pcap, int offset, int count,
     float* volume, int vloc, size_t vcap, int clear_output)
 {
@@ -368,27 +478,49 @@ void yk_fdk::execute(const float* projection, int ploc, size_t pcap, int offset,
     if (vloc == YK_HOST) {
         d_volume_ws.resize(vol_elems);
         d_vol = d_volume_ws.get();
-        if (received == 0 && !clear_output)
-            YK_HIP(hipMemcpyAsync(d_vol, volume, vol_elems * 4, hipMemcpyHostToDevice, s));
+        if (received == 0 && !clear_output) upload_host(d_vol, volume, vol_elems * 4);
     }
-    const int chunk = std::min(batch, count);
+    // Device projections run in batches of `batch` views.  Host projections
+    // run in staged chunks: chunk c is copied into pinned slot c mod 2 by the
+    // host, moved to device slot c mod 2 on the copy stream (once the filter
+    // of chunk c - 2 has read that device slot), and filtered and
+    // backprojected on the plan's stream (once its copy has landed), so the
+    // transfers of later chunks overlap the GPU work of earlier ones.
+    const bool staged = ploc == YK_HOST;
+    const int chunk = std::min(staged ? stage_chunk_views() : batch, count);
+    const size_t chunk_elems = size_t(chunk) * view_elems;
     const size_t eb = storage_bytes(desc.storage);
-    d_filtered_ws.resize(size_t(chunk) * view_elems * eb);
+    d_filtered_ws.resize(chunk_elems * eb);
     if (desc.storage != YK_F32) d_scales_ws.resize(size_t(chunk) * nv);
-    if (ploc == YK_HOST) d_proj_ws.resize(size_t(chunk) * view_elems);
-    if (capture_stages) d_preweighted_ws.resize(size_t(chunk) * view_elems);
+    if (staged) {
+        ensure_staging(chunk_elems * 4);
+        d_proj_ws.resize(2 * chunk_elems);
+    }
+    if (capture_stages) d_preweighted_ws.resize(chunk_elems);
     last_stage = yk_fdk_stage{};
-    for (int base = 0; base < count; base += chunk) {
+    for (int base = 0, c = 0; base < count; base += chunk, ++c) {
         const int k = std::min(chunk, count - base);
+        const int slot = c & 1;
         const float* d_proj = projection + size_t(base) * view_elems;
-        if (ploc == YK_HOST) {
-            YK_HIP(hipMemcpyAsync(d_proj_ws.get(), d_proj, size_t(k) * view_elems * 4,
-                hipMemcpyHostToDevice, s));
-            d_proj = d_proj_ws.get();
+        if (staged) {
+            const size_t bytes = size_t(k) * view_elems * 4;
+            float* d_in = d_proj_ws.get() + size_t(slot) * chunk_elems;
+            if (pinned_used[slot]) YK_HIP(hipEventSynchronize(ev_pinned[slot]));
+            copy_parallel(pinned[slot].get(), d_proj, bytes);
+            if (consumed_used[slot]) YK_HIP(hipStreamWaitEvent(copy_stream, ev_consumed[slot], 0));
+            YK_HIP(hipMemcpyAsync(d_in, pinned[slot].get(), bytes, hipMemcpyHostToDevice, copy_stream));
+            YK_HIP(hipEventRecord(ev_pinned[slot], copy_stream));
+            pinned_used[slot] = true;
+            YK_HIP(hipStreamWaitEvent(s, ev_pinned[slot], 0));
+            d_proj = d_in;
         }
         filter(d_proj, offset + base, k, d_filtered_ws.get(), desc.storage,
             desc.storage == YK_F32 ? nullptr : d_scales_ws.get(),
             capture_stages ? d_preweighted_ws.get() : nullptr);
+        if (staged) {
+            YK_HIP(hipEventRecord(ev_consumed[slot], s));
+            consumed_used[slot] = true;
+        }
         last_stage.first_view = offset + base;
         last_stage.count = k;
         last_stage.d_input = d_proj;
@@ -400,10 +532,10 @@ void yk_fdk::execute(const float* projection, int ploc, size_t pcap, int offset,
         const int acc = (clear_output && base == 0) ? 0 : 1;
         backproject(d_filtered_ws.get(), desc.storage,
             desc.storage == YK_F32 ? nullptr : d_scales_ws.get(), offset + base, k, d_vol, acc);
-        if (ploc == YK_HOST) YK_HIP(hipStreamSynchronize(s));  // staging buffer reuse
     }
     received += count;
-    if (vloc == YK_HOST && received == n_views)
-        YK_HIP(hipMemcpyAsync(volume, d_vol, vol_elems * 4, hipMemcpyDeviceToHost, s));
-    if (vloc == YK_HOST) YK_HIP(hipStreamSynchronize(s));
+    if (vloc == YK_HOST && received == n_views) download_host(volume, d_vol, vol_elems * 4);
+    // host buffers: the call returns with the GPU work done (the caller may
+    // reuse or read them at once)
+    if (vloc == YK_HOST || staged) YK_HIP(hipStreamSynchronize(s));
 }

@@ -50,7 +50,22 @@ struct yk_fdk {
     int lds_quads = 0;
 
     int batch = 0;
-    yk::DeviceArray<float> d_proj_ws;
+    yk::DeviceArray<float> d_proj_ws;     // host projections: two staged chunks
+    // Host buffers move through two library-owned pinned slots: the host
+    // copies a chunk into a slot with several threads while the copy stream
+    // moves the other slot to the device and the plan's stream filters and
+    // backprojects the chunk before (stage_chunk_views()).
+    yk::PinnedBuffer pinned[2];
+    hipStream_t copy_stream = nullptr;   // default priority
+    hipEvent_t ev_pinned[2] = {};        // slot's last transfer done (host may reuse it)
+    hipEvent_t ev_consumed[2] = {};      // filter of the slot's device chunk done
+    bool pinned_used[2] = {};
+    bool consumed_used[2] = {};
+    int stage_chunk_views() const;
+    void ensure_staging(size_t slot_bytes);
+    void upload_host(float* d_dst, const float* host, size_t bytes);
+    void download_host(float* host, const float* d_src, size_t bytes);
+    ~yk_fdk();
     yk::DeviceArray<unsigned char> d_filtered_ws;
     yk::DeviceArray<float> d_scales_ws;
     yk::DeviceArray<float> d_volume_ws;
