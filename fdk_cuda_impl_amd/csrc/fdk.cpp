@@ -209,7 +209,7 @@ void yk_fdk::build_tiles()
 {
     hipStream_t s = ctx->stream;
     const size_t max_lds = 96 * 1024;
-    const size_t quad = desc.storage == YK_F32 ? 16 : 8;
+    const size_t quad = bp_quad_bytes(desc.storage, general ? 1 : 0);
     const int first_bz = bp_block_z(general ? 1 : 0, desc.storage, desc.kahan);
     for (bz = first_bz; bz >= 8; bz = bz > 16 ? 16 : bz / 2) {
         nbx = (grid.nx + 15) / 16;
@@ -268,7 +268,7 @@ void yk_fdk::backproject(const void* d_filtered, int storage, const float* d_sca
     if (!d_filtered || !d_volume) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
     storage_bytes(storage);
     if (storage != YK_F32 && !d_scales) fail(YK_ERR_INVALID_BUFFER, "half / bfloat16 storage needs row scales");
-    const size_t quad = storage == YK_F32 ? 16 : 8;
+    const size_t quad = bp_quad_bytes(storage, general ? 1 : 0);
     if (size_t(lds_quads) * quad > 160 * 1024) fail(YK_ERR_UNSUPPORTED, "footprint exceeds LDS");
     hipStream_t s = ctx->stream;
     int kernel_storage = storage;
@@ -317,7 +317,7 @@ void yk_fdk::backproject_noalloc(const void* d_filtered, int storage, const floa
     if (!d_filtered || !d_volume) fail(YK_ERR_INVALID_BUFFER, "null device buffer");
     storage_bytes(storage);
     if (storage != YK_F32 && !d_scales) fail(YK_ERR_INVALID_BUFFER, "half / bfloat16 storage needs row scales");
-    const size_t quad = storage == YK_F32 ? 16 : 8;
+    const size_t quad = bp_quad_bytes(storage, general ? 1 : 0);
     if (size_t(lds_quads) * quad > 160 * 1024) fail(YK_ERR_UNSUPPORTED, "footprint exceeds LDS");
     const size_t rows = size_t(count) * nv;
     if (storage != YK_F32 && rows > kNoAllocRowFactors)

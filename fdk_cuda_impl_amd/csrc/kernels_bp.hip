@@ -412,8 +412,11 @@ __device__ __forceinline__ half2_t to_half2(unsigned raw, unsigned f)
     }
 }
 
-// Write quad rows [r0, r0 + rows_per) of this lane's column.
-template <int STORAGE>
+// Write quad rows [r0, r0 + rows_per) of this lane's column.  Q32: float
+// quads (always for float storage; half and bfloat16 texels are widened to
+// FP32 here for the general-geometry path, whose per-sample arithmetic is
+// FP32)
+template <int STORAGE, bool Q32>
 __device__ __forceinline__ void commit_tile(void* lds, Prefetch<STORAGE>& pf, const StageMap& m, short4 t)
 {
     if (!m.lane_on) return;
@@ -423,7 +426,7 @@ __device__ __forceinline__ void commit_tile(void* lds, Prefetch<STORAGE>& pf, co
         for (int j = 0; j < kMaxRows; ++j)
             if (j <= rows_per) pf.v[j] = shift_pair<STORAGE>(pf.v[j], m.shift);
     }
-    constexpr int qb = STORAGE == YK_F32 ? 16 : 8;
+    constexpr int qb = Q32 ? 16 : 8;
     const int pitch = lds_pitch(t.z);
     char* dst = static_cast<char*>(lds) + unsigned(__mul24(m.r0, pitch) + m.col) * unsigned(qb);
     const int rowb = pitch * qb;
@@ -432,6 +435,22 @@ __device__ __forceinline__ void commit_tile(void* lds, Prefetch<STORAGE>& pf, co
         for (int j = 0; j + 1 < kMaxRows; ++j) {
             if (j < rows_per) {
                 const float2 a = pf.v[j], b = pf.v[j + 1];
+                *reinterpret_cast<float4*>(dst + j * rowb) = make_float4(a.x, a.y, b.x - a.x, b.y - a.y);
+            }
+        }
+    } else if constexpr (Q32) {
+        float2 x[kMaxRows];
+#pragma unroll
+        for (int j = 0; j < kMaxRows; ++j) {
+            if (j <= rows_per) {
+                const half2_t h = to_half2<STORAGE>(pf.v[j], pf.f[has_factors(STORAGE) ? j : 0]);
+                x[j] = make_float2(float(h.x), float(h.y));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j + 1 < kMaxRows; ++j) {
+            if (j < rows_per) {
+                const float2 a = x[j], b = x[j + 1];
                 *reinterpret_cast<float4*>(dst + j * rowb) = make_float4(a.x, a.y, b.x - a.x, b.y - a.y);
             }
         }
@@ -452,8 +471,9 @@ __device__ __forceinline__ void commit_tile(void* lds, Prefetch<STORAGE>& pf, co
     }
 }
 
-// direct staging for tiles too tall or wide for the register prefetch
-template <int STORAGE>
+// direct staging for tiles too tall or wide for the register prefetch (Q32
+// as in commit_tile)
+template <int STORAGE, bool Q32>
 __device__ void stage_direct(void* lds, __amdgpu_buffer_rsrc_t data, __amdgpu_buffer_rsrc_t fac,
     int nu, short4 t)
 {
@@ -478,10 +498,15 @@ __device__ void stage_direct(void* lds, __amdgpu_buffer_rsrc_t data, __amdgpu_bu
                 xa = to_half2<STORAGE>(a, 0u);
                 xb = to_half2<STORAGE>(b, 0u);
             }
-            uint2 q;
-            q.x = __builtin_bit_cast(unsigned, xa);
-            q.y = __builtin_bit_cast(unsigned, xb - xa);
-            static_cast<uint2*>(lds)[slot] = q;
+            if constexpr (Q32) {
+                const float a0 = float(xa.x), a1 = float(xa.y);
+                static_cast<float4*>(lds)[slot] = make_float4(a0, a1, float(xb.x) - a0, float(xb.y) - a1);
+            } else {
+                uint2 q;
+                q.x = __builtin_bit_cast(unsigned, xa);
+                q.y = __builtin_bit_cast(unsigned, xb - xa);
+                static_cast<uint2*>(lds)[slot] = q;
+            }
         }
     }
 }
@@ -498,6 +523,9 @@ constexpr int kBatch = 8;  // LDS reads issued before their results are consumed
 // to FP32 (relative error below the square of that change) and takes two
 // full-rate instructions instead of the quarter-rate reciprocal.
 constexpr int kRegular = 0, kGeneralRcp = 1, kGeneralNewton = 2;
+
+// float LDS quads: float storage, and every storage on the general path
+constexpr bool bp_float_quads(int storage, int geo) { return storage == YK_F32 || geo != kRegular; }
 
 template <int STORAGE, int BZ, int GEO>
 __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& c, short4 t,
@@ -573,17 +601,15 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
         float un = c.cu[3] + c.cu[0] * X + c.cu[1] * Y + c.cu[2] * Z0;
         float vn = c.cv[3] + c.cv[0] * X + c.cv[1] * Y + c.cv[2] * Z0;
         const float dd = c.cd[2] * vz, du = c.cu[2] * vz, dvn = c.cv[2] * vz;
-        constexpr int qb = STORAGE == YK_F32 ? 16 : 8;
-        const int rowb = rowq * qb;
+        // float quads for every storage (half texels are widened when staged):
+        // this path is VALU-bound, and packed half lerps cost as many cycles
+        // per FMA as FP32 ones on gfx950 plus a conversion per sample
+        const int rowb = rowq * 16;
         // fully unrolled: a partial unroll indexes acc[] through the register
         // index mode (a move in and out per sample).  (Issuing the LDS reads of
         // eight samples before consuming them measured slower: the extra VGPRs
         // cost more than the exposed latency.)
         float r = __builtin_amdgcn_rcpf(den);
-        // half u-weights times the distance weight: (w (1 - fu), w fu) =
-        // fu (-w, w) + (w, 0), so the dot product accumulates straight into acc
-        const half2_t wneg = { (_Float16)(-w), (_Float16)w };
-        const half2_t wlo = { (_Float16)w, (_Float16)0.f };
 #pragma unroll
         for (int k = 0; k < BZ; ++k) {
             if constexpr (GEO == kGeneralRcp) {
@@ -593,24 +619,11 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
             const float u = un * r - ul0;
             const float v = vn * r - vl0;
             const float fu = __builtin_amdgcn_fractf(u), fv = __builtin_amdgcn_fractf(v);
-            const char* qp = static_cast<const char*>(lds) + __mul24(int(v), rowb) + int(u) * qb;
-            if constexpr (STORAGE == YK_F32) {
-                const float4 q = *reinterpret_cast<const float4*>(qp);
-                const float c0 = fmaf(fv, q.z, q.x);
-                const float c1 = fmaf(fv, q.w, q.y);
-                acc[k] = fmaf(w, fmaf(fu, c1 - c0, c0), acc[k]);
-            } else {
-                // packed half lerps: (fv, fu) in one conversion, the v-lerp of
-                // both columns in one FMA, the weighted u-weights in one more,
-                // then a dot product accumulated in FP32
-                const uint2 q = *reinterpret_cast<const uint2*>(qp);
-                const half2_t vu = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(fv, fu));
-                const half2_t fvv = { vu.x, vu.x };
-                const half2_t fuu = { vu.y, vu.y };
-                const half2_t cc = __builtin_bit_cast(half2_t, q.y) * fvv + __builtin_bit_cast(half2_t, q.x);
-                const half2_t wu = fuu * wneg + wlo;
-                acc[k] = __builtin_amdgcn_fdot2(cc, wu, acc[k], false);
-            }
+            const char* qp = static_cast<const char*>(lds) + __mul24(int(v), rowb) + int(u) * 16;
+            const float4 q = *reinterpret_cast<const float4*>(qp);
+            const float c0 = fmaf(fv, q.z, q.x);
+            const float c1 = fmaf(fv, q.w, q.y);
+            acc[k] = fmaf(w, fmaf(fu, c1 - c0, c0), acc[k]);
             den += dd;
             un += du;
             vn += dvn;
@@ -637,6 +650,7 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
 {
     // one LDS tile buffer, or two alternating ones (then a single barrier per
     // view orders the writes of view i+1 after every read of view i-1)
+    constexpr bool Q32 = bp_float_quads(STORAGE, GEO);
     extern __shared__ float4 lds_raw[];
     char* const lds0 = reinterpret_cast<char*>(lds_raw);
     unsigned cur = 0, nxt = double_buffer ? buf_bytes : 0u;
@@ -689,9 +703,9 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
         const StageMap m = stage_map(t_cur, my_stage[0], nu);
         if (m.fast) {
             prefetch_tile<STORAGE>(pf, m, data_rsrc(0), fac_rsrc(0), nu, t_cur);
-            commit_tile<STORAGE>(lds0, pf, m, t_cur);
+            commit_tile<STORAGE, Q32>(lds0, pf, m, t_cur);
         } else {
-            stage_direct<STORAGE>(lds0, data_rsrc(0), fac_rsrc(0), nu, t_cur);
+            stage_direct<STORAGE, Q32>(lds0, data_rsrc(0), fac_rsrc(0), nu, t_cur);
         }
     }
     __syncthreads();
@@ -719,8 +733,8 @@ fdk_bp_kernel(const void* __restrict__ proj, const unsigned* __restrict__ row_fa
         }
         if (t_next.z > 0) {
             if (!double_buffer) __syncthreads();
-            if (m_next.fast) commit_tile<STORAGE>(lds0 + nxt, pf, m_next, t_next);
-            else stage_direct<STORAGE>(lds0 + nxt, data_rsrc(i + 1), fac_rsrc(i + 1), nu, t_next);
+            if (m_next.fast) commit_tile<STORAGE, Q32>(lds0 + nxt, pf, m_next, t_next);
+            else stage_direct<STORAGE, Q32>(lds0 + nxt, data_rsrc(i + 1), fac_rsrc(i + 1), nu, t_next);
             __syncthreads();
             const unsigned t = cur;
             cur = nxt;
@@ -748,7 +762,7 @@ template <int STORAGE, int BZ, int GEO, bool KAHAN>
 void launch_bz(const BpLaunch& p, const int* gbits, hipStream_t s)
 {
     dim3 grid(p.nbx, p.nby, p.nbz), block(kThreads);
-    const size_t buf = size_t(p.lds_quads) * (STORAGE == YK_F32 ? 16 : 8);
+    const size_t buf = size_t(p.lds_quads) * (bp_float_quads(STORAGE, GEO) ? 16 : 8);
     auto k = fdk_bp_kernel<STORAGE, BZ, GEO, KAHAN>;
     // double-buffer the tile when that costs no occupancy: workgroups per CU
     // allowed by the VGPRs (one wave per SIMD each) vs by 160 KB of LDS
@@ -798,11 +812,14 @@ void launch_storage(const BpLaunch& p, const int* gbits, hipStream_t s, int bz)
 
 int bp_block_z(int general, int storage, int kahan)
 {
-    // general path: 16 slices for float quads (32 measured 12 % slower at
-    // 512^3 x 720 with a tilted detector), 32 for half quads (4 % faster)
+    // general path (float quads for every storage): 16 slices (32 measured
+    // 12 % slower at 512^3 x 720 with a tilted detector); regular path: 32
     (void)kahan;
-    return general && storage == YK_F32 ? 16 : 32;
+    (void)storage;
+    return general ? 16 : 32;
 }
+
+size_t bp_quad_bytes(int storage, int general) { return storage == YK_F32 || general ? 16 : 8; }
 
 void launch_bp_tiles(const TileLaunch& p, hipStream_t s)
 {

@@ -240,6 +240,9 @@ void launch_normalize_rows(const void* in, const unsigned* factors, int storage,
 void launch_row_factors(const float* scales, int n, int storage, int* gbits, unsigned* factors,
     hipStream_t s);
 int bp_block_z(int general, int storage, int kahan);
+// bytes of one LDS quad of the backprojection: 16 (float) for float storage
+// and for every storage of a general-geometry plan, 8 (half) otherwise
+size_t bp_quad_bytes(int storage, int general);
 // progress report of the iterative callers: waits for the stream's work, then
 // calls cb(user, completed, total) on this thread; no-op without a callback
 void report_progress(yk_progress_fn cb, void* user, int completed, int total, hipStream_t s);
