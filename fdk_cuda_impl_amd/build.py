@@ -20,6 +20,7 @@ OBJ_DIR = os.path.join(REPO, "_build", "obj")
 LIB_PATH = os.path.join(HERE, "libyk_hip.so")
 SESSION_CHECK = os.path.join(HERE, "yk_session_check")
 VALU_COSTS = os.path.join(HERE, "yk_valu_costs")
+VMEM_COSTS = os.path.join(HERE, "yk_vmem_costs")
 ARCH = os.environ.get("YK_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["capi.cpp", "geometry.cpp", "fdk.cpp", "projector.cpp", "algebraic.cpp", "cgls.cpp", "diagnostics.cpp", "session.cpp",
@@ -65,6 +66,7 @@ def build(jobs: int | None = None) -> str:
     os.replace(tmp, LIB_PATH)
     build_session_check()
     build_valu_costs()
+    build_vmem_costs()
     return LIB_PATH
 
 
@@ -87,12 +89,26 @@ def build_valu_costs() -> str:
     """Compile tests/valu_costs.hip, the VALU instruction-cost probe behind
     profiles/valu_instruction_costs_gfx950.txt, next to the library."""
     src = os.path.join(REPO, "tests", "valu_costs.hip")
-    cmd = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-x", "hip", src, "-o", VALU_COSTS + ".tmp"]
+    cmd = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-x", "hip", src, "-o", VALU_COSTS + ".tmp"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"VALU cost probe build failed\n{res.stdout}\n{res.stderr}")
     os.replace(VALU_COSTS + ".tmp", VALU_COSTS)
     return VALU_COSTS
+
+
+def build_vmem_costs() -> str:
+    """Compile tests/vmem_costs.hip, the vector-memory / LDS cost probe behind
+    profiles/vector_memory_costs_gfx950.txt, next to the library."""
+    src = os.path.join(REPO, "tests", "vmem_costs.hip")
+    cmd = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-x", "hip", src, "-o", VMEM_COSTS + ".tmp"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"vector-memory cost probe build failed\n{res.stdout}\n{res.stderr}")
+    os.replace(VMEM_COSTS + ".tmp", VMEM_COSTS)
+    return VMEM_COSTS
 
 
 if __name__ == "__main__":

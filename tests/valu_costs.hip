@@ -56,6 +56,19 @@ YK_KERNEL(k_mul_lo_u32, "v_mul_lo_u32 %0, %0, %1")
 YK_KERNEL(k_lshl_add, "v_lshl_add_u32 %0, %0, 3, %1")
 YK_KERNEL(k_perm, "v_perm_b32 %0, %0, %1, %2")
 YK_KERNEL(k_rcp_f32, "v_rcp_f32 %0, %0")
+YK_KERNEL(k_fma_mix, "v_fma_mix_f32 %0, %0, %1, %2 op_sel_hi:[0,1,1]")
+YK_KERNEL(k_cvt_f32_f16, "v_cvt_f32_f16 %0, %0")
+YK_KERNEL(k_lshrrev, "v_lshrrev_b32 %0, 3, %0")
+YK_KERNEL(k_and_b32, "v_and_b32 %0, %0, %1")
+YK_KERNEL(k_or_b32, "v_or_b32 %0, %0, %1")
+YK_KERNEL(k_mul_u24, "v_mul_u32_u24 %0, %0, %1")
+YK_KERNEL(k_bfe_u32, "v_bfe_u32 %0, %0, 3, 9")
+YK_KERNEL(k_max_f32, "v_max_f32 %0, %0, %1")
+YK_KERNEL(k_sub_f32, "v_sub_f32 %0, %0, %1")
+YK_KERNEL(k_ldexp_f32, "v_ldexp_f32 %0, %0, 1")
+YK_KERNEL(k_floor_f32, "v_floor_f32 %0, %0")
+YK_KERNEL(k_med3_f32, "v_med3_f32 %0, %0, %1, %2")
+YK_KERNEL(k_sqrt_f32, "v_sqrt_f32 %0, %0")
 
 using Kernel = void (*)(unsigned*, unsigned);
 
@@ -77,7 +90,11 @@ int main()
         { "v_cvt_pkrtz_f16_f32", k_cvt_pkrtz }, { "v_pk_fma_f16", k_pk_fma_f16 }, { "v_pk_add_f16", k_pk_add_f16 },
         { "v_dot2c_f32_f16", k_dot2c }, { "v_mad_u32_u24", k_mad_u24 }, { "v_mad_u32_u16", k_mad_u16 },
         { "v_mul_lo_u32", k_mul_lo_u32 }, { "v_lshl_add_u32", k_lshl_add }, { "v_perm_b32", k_perm },
-        { "v_rcp_f32", k_rcp_f32 },
+        { "v_rcp_f32", k_rcp_f32 }, { "v_fma_mix_f32", k_fma_mix }, { "v_cvt_f32_f16", k_cvt_f32_f16 },
+        { "v_lshrrev_b32", k_lshrrev }, { "v_and_b32", k_and_b32 }, { "v_or_b32", k_or_b32 },
+        { "v_mul_u32_u24", k_mul_u24 }, { "v_bfe_u32", k_bfe_u32 }, { "v_max_f32", k_max_f32 },
+        { "v_sub_f32", k_sub_f32 }, { "v_ldexp_f32", k_ldexp_f32 }, { "v_floor_f32", k_floor_f32 },
+        { "v_med3_f32", k_med3_f32 }, { "v_sqrt_f32", k_sqrt_f32 },
     };
     std::printf("CUs %d, clock %.3f GHz\n", cus, ghz);
     hipEvent_t e0, e1;
