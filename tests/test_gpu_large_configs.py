@@ -11,6 +11,8 @@
   the whole scan FP16 against FP32), and 10 CGLS iterations through the
   differentiable operators.
 """
+import time
+
 import numpy as np
 import pytest
 import torch
@@ -110,7 +112,12 @@ def c5(ctx):
     P = yk.Projector(ctx, g.views(), g.nu, g.nv, g.grid())
     ell = phantoms.random_ellipsoids(5, 256, semi_axes=(5.0, 40.0), density=(0.0, 0.05), center_radius=50.0)
     truth = ctx.phantom(ell, g.grid(), clamp=False)
+    P.forward(truth, first=0, count=8)  # paired volume copies and first-call setup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     proj = P.forward(truth)
+    torch.cuda.synchronize()
+    print(f"1024^3 x 1440 Joseph forward projection: {time.perf_counter() - t0:.2f} s", flush=True)
     yield g, P, truth, proj
     P.close()
 
@@ -167,7 +174,11 @@ def test_c5_fdk_fp32_and_fp16(ctx, oracle, c5):
 
 def test_c5_cgls_10_iterations_through_autograd(c5):
     g, P, truth, proj = c5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     x, res = ag.cgls(P, proj, iterations=10)
+    torch.cuda.synchronize()
+    print(f"CGLS 10 at 1024^3 x 1440: {time.perf_counter() - t0:.1f} s", flush=True)
     err = float(torch.linalg.vector_norm(x - truth) / torch.linalg.vector_norm(truth))
     print(f"CGLS 10 at 1024^3 x 1440: ||r|| {res[0]:.4e} -> {res[-1]:.4e}, rel error vs phantom {err:.3f}")
     assert len(res) == 11
