@@ -149,6 +149,7 @@ yk_fdk::yk_fdk(yk_ctx* c, const yk_fdk_desc& d, const yk_view* v, int n, int nu_
     // filter: weights use du of the first view (YkFDKFilterProcessor.hpp:111)
     n_pad = padded_n(nu);
     filter_weights(desc.filter, n_pad, derived[0].du, weights);
+    response_l1 = float(filter_response_l1(weights, n_pad));
     d_weights.upload(weights.data(), weights.size(), s);
     std::vector<float2> tw;
     fft_twiddles(n_pad, tw);
@@ -256,6 +257,7 @@ void yk_fdk::filter(const float* d_proj, int first, int count, void* d_out, int 
     p.weights = d_weights.get();
     p.twiddles = d_twiddles.get();
     p.nu = nu; p.nv = nv; p.count = count; p.n = n_pad;
+    p.response_l1 = response_l1;
     p.storage = storage;
     launch_fdk_filter(p, ctx->stream);
     if (d_preweighted) launch_preweight(p, d_preweighted, ctx->stream);

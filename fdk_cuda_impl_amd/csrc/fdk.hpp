@@ -34,6 +34,7 @@ struct yk_fdk {
     std::vector<yk::CoeffD> coeffs;
     std::vector<float> weights;
     int n_pad = 0;
+    float response_l1 = 0.f;       // filter_response_l1 of weights
 
     yk::DeviceArray<float> d_weights;
     yk::DeviceArray<float2> d_twiddles;

@@ -348,6 +348,28 @@ void filter_weights(const yk_filter& f, int n, double du, std::vector<float>& ou
     }
 }
 
+double filter_response_l1(const std::vector<float>& weights, int n)
+{
+    if (n < 2 || int(weights.size()) != n / 2 + 1) fail(YK_ERR_INVALID_CONFIG, "filter weights of another length");
+    // cos(2 pi j / n) for j in [0, n): the argument k m reduces modulo n
+    std::vector<double> c(n);
+    for (int j = 0; j < n; ++j) c[j] = std::cos(2.0 * M_PI * j / n);
+    double l1 = 0.0;
+    for (int m = 0; m <= n / 2; ++m) {
+        // h[m] = w[0] + (-1)^m w[n/2] + 2 sum_{0<k<n/2} w[k] cos(2 pi k m / n)
+        double h = double(weights[0]) + ((m & 1) ? -1.0 : 1.0) * double(weights[n / 2]);
+        size_t idx = 0;
+        for (int k = 1; k < n / 2; ++k) {
+            idx += size_t(m);
+            if (idx >= size_t(n)) idx -= size_t(n);
+            h += 2.0 * double(weights[k]) * c[idx];
+        }
+        // h is even: h[n - m] = h[m]
+        l1 += (m == 0 || m == n / 2) ? std::fabs(h) : 2.0 * std::fabs(h);
+    }
+    return l1;
+}
+
 VolumeGeom volume_geom(const yk_volume_grid& g)
 {
     VolumeGeom v{};

@@ -312,27 +312,29 @@ __device__ __forceinline__ float row_weight(const RowWeight& w, int u)
     return x;
 }
 
-// workgroup maxima of a and b (both rows of a pair at once)
-__device__ __forceinline__ void block_max2(float& a, float& b, float2* red)
+// wave maxima of a and b (both rows of a pair at once), stored to red[wave]
+// for the reads after the next workgroup barrier (wave_max2_read)
+__device__ __forceinline__ void wave_max2_store(float a, float b, float2* red)
 {
-    // wave64 reduction through shuffles, then across waves
     for (int off = 32; off > 0; off >>= 1) {
         a = fmaxf(a, __shfl_xor(a, off, 64));
         b = fmaxf(b, __shfl_xor(b, off, 64));
     }
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) red[wave] = make_float2(a, b);
-    __syncthreads();
-    float2 m = red[0];
-    for (int w = 1; w < kFilterThreads / 64; ++w) m = make_float2(fmaxf(m.x, red[w].x), fmaxf(m.y, red[w].y));
-    __syncthreads();
-    a = m.x;
-    b = m.y;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_float2(a, b);
 }
 
-// power-of-two scale s = 2^e with max|row| / s in (2^13, 2^14]; e = 0 for
-// an all-zero (or non-finite) row.  Returns e: s and 1 / s are exact powers
-// of two.
+__device__ __forceinline__ float2 wave_max2_read(const float2* red)
+{
+    float2 m = red[0];
+    for (int w = 1; w < kFilterThreads / 64; ++w) m = make_float2(fmaxf(m.x, red[w].x), fmaxf(m.y, red[w].y));
+    return m;
+}
+
+// power-of-two scale s = 2^e with bound / s in (2^13, 2^14], where bound
+// >= max|row| (the largest weighted input texel times the filter's response
+// L1 norm); e = 0 for an all-zero (or non-finite) row.  Returns e: s and 1 / s
+// are exact powers of two.  Half and bfloat16 keep their relative precision
+// at any such scale, so a loose bound only moves the row's exponent range.
 __device__ __forceinline__ int row_scale_exp(float maxabs)
 {
     if (!(maxabs > 0.f) || !isfinite(maxabs)) return 0;
@@ -383,7 +385,7 @@ __global__ void __launch_bounds__(kFilterThreads, NPT >= 16 ? 2 : 4)
 fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
     float* __restrict__ row_scales, const FilterView* __restrict__ views,
     const float* __restrict__ parker, const float* __restrict__ weights,
-    const float2* __restrict__ tw, int nu, int nv, int rows, int n)
+    const float2* __restrict__ tw, int nu, int nv, int rows, int n, float response_l1)
 {
     extern __shared__ float4 smem_raw[];
     const int padded = pidx(n);
@@ -447,6 +449,15 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
                 const float wb = (has_b && ub >= 0 && ub < nu) ? row_weight(wtb, ub) : 0.f;
                 wv[e] = make_float2(xa[e] * wa, xb[e] * wb);
             }
+            if constexpr (STORAGE != YK_F32) {
+                float ma = 0.f, mb = 0.f;
+#pragma unroll
+                for (int e = 0; e < NPT; ++e) {
+                    ma = fmaxf(ma, fabsf(wv[e].x));
+                    mb = fmaxf(mb, fabsf(wv[e].y));
+                }
+                wave_max2_store(ma, mb, red);
+            }
 #pragma unroll
             for (int t = 0; t < S; ++t) {
                 float2 v[8];
@@ -461,6 +472,7 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
             spec = fft_forward_ct<N, 8, true>(bufb, bufa, twr, wr, tw_lds);
         } else {
             // cosine (and Parker) weights on the way into LDS
+            float ma = 0.f, mb = 0.f;
 #pragma unroll
             for (int e = 0; e < NPT; ++e) {
                 const int m = int(threadIdx.x) + e * kFilterThreads;
@@ -468,9 +480,13 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
                     const int ua = m - fa.start_u, ub = m - fb.start_u;
                     const float wa = (ua >= 0 && ua < nu) ? row_weight(wta, ua) : 0.f;
                     const float wb = (has_b && ub >= 0 && ub < nu) ? row_weight(wtb, ub) : 0.f;
-                    bufa[io_slot<NPT>(m)] = make_float2(xa[e] * wa, xb[e] * wb);
+                    const float2 x = make_float2(xa[e] * wa, xb[e] * wb);
+                    ma = fmaxf(ma, fabsf(x.x));
+                    mb = fmaxf(mb, fabsf(x.y));
+                    bufa[io_slot<NPT>(m)] = x;
                 }
             }
+            if constexpr (STORAGE != YK_F32) wave_max2_store(ma, mb, red);
             __syncthreads();
             if constexpr (NPT > 1) {
                 spec = fft_forward_ct<NC, 1, true>(bufa, bufb, twr, wr, tw_lds);
@@ -492,14 +508,10 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
         // crop: row a = Re(res), row b = -Im(res)
         float inv_a = 1.f, inv_b = 1.f;
         if constexpr (STORAGE != YK_F32) {
-            float ma = 0.f, mb = 0.f;
-            for (int a = threadIdx.x; a < nu; a += kFilterThreads) {
-                const int ja = fa.start_u + a, jb = fb.start_u + a;
-                if (ja >= 0 && ja < n) ma = fmaxf(ma, fabsf(res[io_slot<NPT>(ja)].x));
-                if (jb >= 0 && jb < n) mb = fmaxf(mb, fabsf(res[io_slot<NPT>(jb)].y));
-            }
-            block_max2(ma, mb, red);
-            const int ea = row_scale_exp(ma), eb = row_scale_exp(mb);
+            // red holds the wave maxima of the weighted inputs, stored before
+            // the first barrier of this pair's transforms
+            const float2 mx = wave_max2_read(red);
+            const int ea = row_scale_exp(mx.x * response_l1), eb = row_scale_exp(mx.y * response_l1);
             const float sa = ldexpf(1.f, ea), sb = ldexpf(1.f, eb);
             inv_a = ldexpf(1.f, -ea);
             inv_b = ldexpf(1.f, -eb);
@@ -539,7 +551,7 @@ void launch_npt(const FilterLaunch& p, int rows, size_t lds, hipStream_t s)
     const int pairs = (rows + 1) / 2;
     const int grid = std::max(1, std::min(pairs, cus * std::min(per_cu, 8)));
     hipLaunchKernelGGL(k, dim3(grid), dim3(kFilterThreads), lds, s, p.proj, p.out, p.row_scales, p.views,
-        p.parker, p.weights, p.twiddles, p.nu, p.nv, rows, p.n);
+        p.parker, p.weights, p.twiddles, p.nu, p.nv, rows, p.n, p.response_l1);
 }
 
 template <int STORAGE>

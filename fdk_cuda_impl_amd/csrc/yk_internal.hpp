@@ -136,6 +136,10 @@ void derive_geometry(const std::vector<yk_view>& views, int nu, int nv, double s
 void fdk_coefficients(const std::vector<yk_view>& views, const std::vector<DerivedD>& derived,
     std::vector<CoeffD>& out);
 void filter_weights(const yk_filter& f, int padded_n, double du, std::vector<float>& out);
+// sum of |h[m]| over the padded_n-periodic impulse response h[m] =
+// sum_k w[|k|] cos(2 pi k m / padded_n) of the filter weights (FP64): a
+// filtered row never exceeds this times the largest |texel| of its input
+double filter_response_l1(const std::vector<float>& weights, int padded_n);
 void validate_filter(const yk_filter& f);
 int padded_n(int nu);
 void fft_twiddles(int n, std::vector<float2>& out);
@@ -190,6 +194,7 @@ struct FilterLaunch {
     const float2* twiddles;     // device, N
     int nu, nv, count, n;
     int storage;
+    float response_l1;          // filter_response_l1 of `weights` (half / bf16 row scales)
 };
 void launch_fdk_filter(const FilterLaunch& p, hipStream_t s);
 // the cosine (and Parker) weighted projections of p's views into out ([count][nv][nu])
