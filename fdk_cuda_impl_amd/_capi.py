@@ -24,7 +24,10 @@ FP_JOSEPH, FP_SIDDON = 1, 2
 BP_JOSEPH_ADJOINT, BP_SIDDON_ADJOINT, BP_FDK, BP_FDK_PSEUDO_MATCHED, BP_JOSEPH_VOXEL = 8, 4, 6, 7, 10
 PARKER_AUTO, PARKER_DISABLED, PARKER_ENABLED = 0, 1, 2
 FILTERS = {"none": 0, "ramlak": 1, "shepplogan": 2, "cosine": 3, "hann": 4, "hamming": 5,
-           "blackman": 6, "butterworth": 7, "kaiser": 8, "tukey": 9}
+           "blackman": 6, "butterworth": 7, "kaiser": 8, "tukey": 9, "custom": 10}
+# yk_filter_source / yk_ramp_extract
+FILTER_SOURCES = {"analytic": 0, "discrete_rl": 1, "spatial_ramp": 2}
+RAMP_EXTRACT = {"real": 0, "magnitude": 1}
 # YK_NOALLOC_MAX_ROWS: row factors (views x detector rows) one call of
 # yk_fdk_backproject_noalloc may use
 NOALLOC_MAX_ROWS = 4194304
@@ -79,7 +82,9 @@ class System(ctypes.Structure):
 class Filter(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int), ("cutoff", ctypes.c_float), ("gain", ctypes.c_float),
                 ("butterworth_order", ctypes.c_float), ("kaiser_beta", ctypes.c_float),
-                ("tukey_alpha", ctypes.c_float)]
+                ("tukey_alpha", ctypes.c_float), ("source", ctypes.c_int), ("extract_mode", ctypes.c_int),
+                ("spatial_ramp", ctypes.POINTER(ctypes.c_float)), ("spatial_ramp_size", ctypes.c_int),
+                ("custom_weights", ctypes.POINTER(ctypes.c_float)), ("custom_weights_size", ctypes.c_int)]
 
 
 class FdkDesc(ctypes.Structure):

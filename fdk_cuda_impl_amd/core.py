@@ -135,11 +135,27 @@ def fdk_coefficients(views: np.ndarray, nu: int, nv: int, scan_range: float = 0.
 
 
 def make_filter(kind="ramlak", cutoff=0.5, gain=1.0, butterworth_order=2.0, kaiser_beta=8.6,
-                tukey_alpha=0.5) -> C.Filter:
+                tukey_alpha=0.5, source="analytic", extract_mode="real", spatial_ramp=None,
+                custom_weights=None) -> C.Filter:
+    """yk_filter.  source: "analytic" | "discrete_rl" | "spatial_ramp" (taps in
+    spatial_ramp); kind "custom" takes N/2 + 1 frequency weights.  The arrays
+    are kept on the returned struct (attribute ``keep``) for the C call."""
     f = C.Filter()
     f.type = C.FILTERS[kind] if isinstance(kind, str) else int(kind)
     f.cutoff, f.gain = cutoff, gain
     f.butterworth_order, f.kaiser_beta, f.tukey_alpha = butterworth_order, kaiser_beta, tukey_alpha
+    f.source = C.FILTER_SOURCES[source] if isinstance(source, str) else int(source)
+    f.extract_mode = C.RAMP_EXTRACT[extract_mode] if isinstance(extract_mode, str) else int(extract_mode)
+    keep = []
+    if spatial_ramp is not None:
+        a = np.ascontiguousarray(spatial_ramp, dtype=np.float32)
+        keep.append(a)
+        f.spatial_ramp, f.spatial_ramp_size = a.ctypes.data_as(C._FP), a.size
+    if custom_weights is not None:
+        a = np.ascontiguousarray(custom_weights, dtype=np.float32)
+        keep.append(a)
+        f.custom_weights, f.custom_weights_size = a.ctypes.data_as(C._FP), a.size
+    f.keep = keep
     return f
 
 
@@ -254,6 +270,10 @@ class FdkOptions:
     butterworth_order: float = 2.0
     kaiser_beta: float = 8.6
     tukey_alpha: float = 0.5
+    filter_source: str = "analytic"
+    extract_mode: str = "real"
+    spatial_ramp: np.ndarray | None = None
+    custom_weights: np.ndarray | None = None
     parker: str = "auto"
     scan_range_rad: float = 0.0
     start_angle_rad: float = 0.0
@@ -265,8 +285,11 @@ class FdkOptions:
     def desc(self) -> C.FdkDesc:
         d = C.FdkDesc()
         d.struct_size = ctypes.sizeof(C.FdkDesc)
-        d.filter = make_filter(self.filter, self.cutoff, self.gain, self.butterworth_order,
-                               self.kaiser_beta, self.tukey_alpha)
+        f = make_filter(self.filter, self.cutoff, self.gain, self.butterworth_order,
+                        self.kaiser_beta, self.tukey_alpha, self.filter_source, self.extract_mode,
+                        self.spatial_ramp, self.custom_weights)
+        d.filter = f
+        d.keep = f.keep  # the arrays d.filter points at
         d.parker_mode = {"auto": C.PARKER_AUTO, "disabled": C.PARKER_DISABLED,
                          "enabled": C.PARKER_ENABLED}[self.parker]
         d.scan_range_rad = self.scan_range_rad

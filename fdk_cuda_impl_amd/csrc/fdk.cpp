@@ -150,6 +150,9 @@ yk_fdk::yk_fdk(yk_ctx* c, const yk_fdk_desc& d, const yk_view* v, int n, int nu_
     n_pad = padded_n(nu);
     filter_weights(desc.filter, n_pad, derived[0].du, weights);
     response_l1 = float(filter_response_l1(weights, n_pad));
+    // the caller's spatial ramp / custom weights are read only here
+    desc.filter.spatial_ramp = nullptr;
+    desc.filter.custom_weights = nullptr;
     d_weights.upload(weights.data(), weights.size(), s);
     std::vector<float2> tw;
     fft_twiddles(n_pad, tw);

@@ -263,12 +263,23 @@ int main_axis(const yk_view& v, int nu, int nv)
 // validAnalyticFilter (src/Filter/YkAnalyticFilterSettings.hpp:6-13)
 void validate_filter(const yk_filter& p)
 {
-    if (p.type < YK_FILTER_NONE || p.type > YK_FILTER_TUKEY || !std::isfinite(p.cutoff) ||
+    if (p.type < YK_FILTER_NONE || p.type > YK_FILTER_CUSTOM || !std::isfinite(p.cutoff) ||
         !(p.cutoff > 0.f) || !(p.cutoff <= 0.5f) || !std::isfinite(p.gain) || !(p.gain > 0.f) ||
         !std::isfinite(p.butterworth_order) || !(p.butterworth_order > 0.f) ||
         !std::isfinite(p.kaiser_beta) || !(p.kaiser_beta >= 0.f) ||
         !std::isfinite(p.tukey_alpha) || !(p.tukey_alpha >= 0.f) || !(p.tukey_alpha <= 1.f))
         fail(YK_ERR_INVALID_CONFIG, "invalid analytic filter parameters");
+    if (p.source < YK_FILTER_SOURCE_ANALYTIC_FREQ || p.source > YK_FILTER_SOURCE_SPATIAL_RAMP_FFT)
+        fail(YK_ERR_INVALID_CONFIG, "unknown filter weight source");
+    if (p.extract_mode != YK_RAMP_EXTRACT_REAL && p.extract_mode != YK_RAMP_EXTRACT_MAGNITUDE)
+        fail(YK_ERR_INVALID_CONFIG, "unknown ramp extract mode");
+    if (p.source == YK_FILTER_SOURCE_SPATIAL_RAMP_FFT) {
+        if (!p.spatial_ramp || p.spatial_ramp_size <= 0 || (p.spatial_ramp_size & 1) == 0)
+            fail(YK_ERR_INVALID_CONFIG, "spatial ramp needs an odd number of taps");
+    } else if (p.type == YK_FILTER_CUSTOM) {
+        if (!p.custom_weights || p.custom_weights_size <= 0)
+            fail(YK_ERR_INVALID_CONFIG, "custom filter without weights");
+    }
 }
 
 namespace {
@@ -290,9 +301,14 @@ double bessel_i0(double x)
 }
 
 // window_shape_desc (src/Filter/YkCreateFilterKernelHelpers.cuh:38-103)
-double window(const yk_filter& f, double x)
+// window_shape_desc (parametric: Butterworth, Kaiser and Tukey use their
+// parameters) or window_shape (those three give 1), the forms the analytic and
+// the DFT-built sources use (src/Filter/YkCreateFilterKernelHelpers.cuh:38-103)
+double window(const yk_filter& f, double x, bool parametric = true)
 {
     x = std::min(std::max(x, 0.0), 1.0);
+    if (!parametric && (f.type == YK_FILTER_BUTTERWORTH || f.type == YK_FILTER_KAISER || f.type == YK_FILTER_TUKEY))
+        return 1.0;
     switch (f.type) {
     case YK_FILTER_RAMLAK: return 1.0;
     case YK_FILTER_SHEPPLOGAN: {
@@ -323,12 +339,46 @@ double window(const yk_filter& f, double x)
     }
 }
 
+// Bins 0..n/2 of the forward DFT of a real n-periodic sequence given by its
+// nonzero taps (index, value), as the weight the extract mode takes: the real
+// part or the magnitude (kernel_extract_weights_from_fft,
+// src/Filter/YkCreateFilterKernelKernels.cu:299-320), in FP64.
+void dft_extract(const std::vector<std::pair<int, double>>& taps, int n, int mode, std::vector<double>& w)
+{
+    std::vector<double> c(n), s(n);
+    for (int j = 0; j < n; ++j) {
+        c[j] = std::cos(2.0 * kPi * j / n);
+        s[j] = std::sin(2.0 * kPi * j / n);
+    }
+    const int nc = n / 2 + 1;
+    w.assign(nc, 0.0);
+    for (int k = 0; k < nc; ++k) {
+        double re = 0.0, im = 0.0;
+        for (const auto& t : taps) {
+            const int j = int((long long)k * t.first % n);
+            re += t.second * c[j];
+            im -= t.second * s[j];
+        }
+        w[k] = mode == YK_RAMP_EXTRACT_MAGNITUDE ? std::hypot(re, im) : re;
+    }
+}
+
 }  // namespace
 
-// AnalyticFreq weights with 1/N baked in, then 1/du
-// (kernel_build_weights_analytic_freq + kernel_scale_inplace,
-// src/Filter/YkCreateFilterKernelKernels.cu:135-192; build_weights
-// src/Filter/YkCreateFilterKernelFFT.hpp:175-190)
+// Frequency weights with 1/N baked in, restating build_weights and
+// build_custom_weights (src/Filter/YkCreateFilterKernelFFT.hpp:114-258) and
+// the kernels they launch (src/Filter/YkCreateFilterKernelKernels.cu:135-345),
+// in FP64:
+//   SpatialRampFFT (any kind): the taps centred on index 0 of an N-periodic
+//     row, / N, DFT, extract; unless Ram-Lak, zero above the cutoff and x
+//     gain window_shape(f / cutoff); DC := 0; x gain / du (so a windowed
+//     spatial ramp carries the gain twice, as the reference's does).
+//   Custom: the caller's N/2 + 1 weights x gain / N / du.
+//   None: gain / N.
+//   AnalyticFreq: gain f window_shape_desc(f / cutoff) / N / du, f <= cutoff.
+//   DiscreteRLFFT: the du = 1 sampled Ram-Lak kernel (1/4 at 0, -1/(pi^2 n^2)
+//     at odd n, wrapped at N/2) / N, DFT, extract, DC := 0, zero above the
+//     cutoff and x gain window_shape(f / cutoff), / du.
 void filter_weights(const yk_filter& f, int n, double du, std::vector<float>& out)
 {
     validate_filter(f);
@@ -336,17 +386,68 @@ void filter_weights(const yk_filter& f, int n, double du, std::vector<float>& ou
     if (!(du > 0.0)) fail(YK_ERR_INVALID_GEOMETRY, "detector pixel size must be positive");
     const int nc = n / 2 + 1;
     out.assign(nc, 0.f);
-    if (f.type == YK_FILTER_NONE) {
-        for (int k = 0; k < nc; ++k) out[k] = float(double(f.gain) / n);
+    const double gain = f.gain, cc = f.cutoff > 0.f ? f.cutoff : 0.5;
+    if (f.source == YK_FILTER_SOURCE_SPATIAL_RAMP_FFT) {
+        const int m = f.spatial_ramp_size;
+        if (m > n) fail(YK_ERR_INVALID_CONFIG, "spatial ramp longer than the padded row");
+        for (int i = 0; i < m; ++i)
+            if (!std::isfinite(f.spatial_ramp[i])) fail(YK_ERR_INVALID_CONFIG, "spatial ramp tap not finite");
+        for (int i = 0; i < m / 2; ++i)
+            if (!(std::fabs(f.spatial_ramp[i] - f.spatial_ramp[m - 1 - i]) < 1e-4f))
+                fail(YK_ERR_INVALID_CONFIG, "spatial ramp not symmetric");
+        std::vector<std::pair<int, double>> taps;
+        for (int j = 0; j < m; ++j) {
+            int idx = j - m / 2;
+            if (idx < 0) idx += n;
+            taps.emplace_back(idx, double(f.spatial_ramp[j]) / n);
+        }
+        std::vector<double> w;
+        dft_extract(taps, n, f.extract_mode, w);
+        if (f.type != YK_FILTER_RAMLAK) {
+            for (int k = 0; k < nc; ++k) {
+                const double fr = double(k) / n;
+                w[k] = fr > cc ? 0.0 : w[k] * gain * window(f, fr / cc, false);
+            }
+        }
+        w[0] = 0.0;
+        for (int k = 0; k < nc; ++k) out[k] = float(w[k] * gain / du);
         return;
     }
-    const double cc = f.cutoff > 0.f ? f.cutoff : 0.5;
+    if (f.type == YK_FILTER_CUSTOM) {
+        if (f.custom_weights_size != nc) fail(YK_ERR_INVALID_CONFIG, "custom filter needs padded N / 2 + 1 weights");
+        for (int k = 0; k < nc; ++k) {
+            if (!std::isfinite(f.custom_weights[k])) fail(YK_ERR_INVALID_CONFIG, "custom filter weight not finite");
+            out[k] = float(double(f.custom_weights[k]) * gain / n / du);
+        }
+        return;
+    }
+    if (f.type == YK_FILTER_NONE) {
+        for (int k = 0; k < nc; ++k) out[k] = float(gain / n);
+        return;
+    }
+    if (f.source == YK_FILTER_SOURCE_ANALYTIC_FREQ) {
+        for (int k = 0; k < nc; ++k) {
+            const double fr = double(k) / n;
+            const double w = fr > cc ? 0.0 : gain * fr * window(f, fr / cc) / n;
+            out[k] = float(w / du);
+        }
+        return;
+    }
+    std::vector<std::pair<int, double>> taps;
+    taps.emplace_back(0, 0.25 / n);
+    for (int u = 1; u < n; ++u) {
+        const int nn = u <= n / 2 ? u : u - n;
+        if (nn & 1) taps.emplace_back(u, -1.0 / (kPi * kPi * double(nn) * double(nn)) / n);
+    }
+    std::vector<double> w;
+    dft_extract(taps, n, f.extract_mode, w);
+    w[0] = 0.0;
     for (int k = 0; k < nc; ++k) {
         const double fr = double(k) / n;
-        const double w = fr > cc ? 0.0 : double(f.gain) * fr * window(f, fr / cc) / n;
-        out[k] = float(w / du);
+        out[k] = float((fr > cc ? 0.0 : w[k] * gain * window(f, fr / cc, false)) / du);
     }
 }
+
 
 double filter_response_l1(const std::vector<float>& weights, int n)
 {

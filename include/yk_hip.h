@@ -63,8 +63,24 @@ typedef enum yk_filter_type {
     YK_FILTER_NONE = 0, YK_FILTER_RAMLAK = 1, YK_FILTER_SHEPPLOGAN = 2,
     YK_FILTER_COSINE = 3, YK_FILTER_HANN = 4, YK_FILTER_HAMMING = 5,
     YK_FILTER_BLACKMAN = 6, YK_FILTER_BUTTERWORTH = 7, YK_FILTER_KAISER = 8,
-    YK_FILTER_TUKEY = 9
+    YK_FILTER_TUKEY = 9,
+    /* caller-given frequency weights (EFilterKernel::Custom, src/global/YkFilterTypes.hpp:8-20) */
+    YK_FILTER_CUSTOM = 10
 } yk_filter_type;
+
+/* YK::EWeightsBuildSource (src/global/YkFilterTypes.hpp:22-26): how the
+ * frequency weights of a windowed ramp are built */
+typedef enum yk_filter_source {
+    YK_FILTER_SOURCE_ANALYTIC_FREQ = 0,     /* gain |f| W(f / cutoff) */
+    YK_FILTER_SOURCE_DISCRETE_RL_FFT = 1,   /* DFT of the sampled Ram-Lak kernel, windowed */
+    YK_FILTER_SOURCE_SPATIAL_RAMP_FFT = 2   /* DFT of caller-given spatial taps, windowed */
+} yk_filter_source;
+
+/* YK::ERampExtractMode (src/global/YkFilterTypes.hpp:28-31): the weight a
+ * DFT bin gives */
+typedef enum yk_ramp_extract {
+    YK_RAMP_EXTRACT_REAL = 0, YK_RAMP_EXTRACT_MAGNITUDE = 1
+} yk_ramp_extract;
 
 /* YK::EParkerMode (algorithms/analytic/YkFdkParams.hpp) */
 typedef enum yk_parker_mode {
@@ -146,7 +162,11 @@ typedef struct yk_system {
     yk_volume_grid volume;
 } yk_system;
 
-/* SAnalyticFilterParams (algorithms/analytic/YkAnalyticFilterParams.hpp:9-14) */
+/* SAnalyticFilterParams (algorithms/analytic/YkAnalyticFilterParams.hpp:9-14),
+ * with the weight sources of SFilterKernelDesc (src/global/YkFilterTypes.hpp:34-44).
+ * A zero tail (source, extract_mode, null arrays) is the analytic filter.
+ * The host arrays are read when the weights are built (plan creation,
+ * yk_filter_weights) and not kept. */
 typedef struct yk_filter {
     int type;  /* yk_filter_type */
     float cutoff;
@@ -154,6 +174,13 @@ typedef struct yk_filter {
     float butterworth_order;
     float kaiser_beta;
     float tukey_alpha;
+    int source;                  /* yk_filter_source */
+    int extract_mode;            /* yk_ramp_extract (DiscreteRL / SpatialRamp sources) */
+    const float* spatial_ramp;   /* SpatialRamp source: odd number of taps, centre tap
+                                    at index size / 2, symmetric to 1e-4 */
+    int spatial_ramp_size;       /* <= padded N */
+    const float* custom_weights; /* YK_FILTER_CUSTOM: padded N / 2 + 1 weights of bins 0..N/2 */
+    int custom_weights_size;
 } yk_filter;
 
 /* FDK plan options (SFdkAlgoParams + scan range, plus the MI355X storage
@@ -219,8 +246,8 @@ yk_status yk_geometry_fdk_coefficients(const yk_view* views, int n_views, int nu
     int nv, float scan_range_rad, float* out);
 
 /* Ramp-filter frequency weights w[0..N/2] with 1/N baked in
- * (CreateFilterKernelFromFFT::build_weights, src/Filter/YkCreateFilterKernelFFT.hpp:114-258,
- * AnalyticFreq source).  N must be a power of two. */
+ * (CreateFilterKernelFromFFT::build_weights, src/Filter/YkCreateFilterKernelFFT.hpp:114-258:
+ * every source and the Custom and None kinds).  N must be a power of two. */
 yk_status yk_filter_weights(const yk_filter* filter, int padded_n, float du_mm,
     float* out);
 /* smallest power of two >= 2 nu (.../FDK/kernels/YkFDKFilterHelpers.cuh:29-35) */

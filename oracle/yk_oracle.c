@@ -448,22 +448,70 @@ double yko_window(int kind, double x, double order, double beta, double alpha,
     }
 }
 
+/* Bins 0..N/2 of the forward DFT of the N-periodic real sequence h, as its
+ * real part (mode 0) or magnitude (mode 1)
+ * (kernel_extract_weights_from_fft, src/Filter/YkCreateFilterKernelKernels.cu:299-320). */
+static void yko_dft_extract(const double* h, int N, int mode, double* w)
+{
+    for (int k = 0; k <= N / 2; ++k) {
+        double re = 0.0, im = 0.0;
+        for (int u = 0; u < N; ++u) {
+            if (h[u] == 0.0) continue;
+            const double a = 2.0 * M_PI * (double)(((long long)k * u) % N) / (double)N;
+            re += h[u] * cos(a);
+            im -= h[u] * sin(a);
+        }
+        w[k] = mode == 1 ? sqrt(re * re + im * im) : re;
+    }
+}
+
 /*
  * Frequency weights w[k], k = 0..N/2, with 1/N baked in
- * (restates CreateFilterKernelFromFFT::build_weights,
+ * (restates CreateFilterKernelFromFFT::build_weights and build_custom_weights,
  * src/Filter/YkCreateFilterKernelFFT.hpp:114-258, and the kernels in
  * src/Filter/YkCreateFilterKernelKernels.cu:135-345).
- *   AnalyticFreq: w = gain f W(f/c) / N / du for f = k/N <= c, else 0.
- *   DiscreteRLFFT: Re DFT of the du=1 spatial Ram-Lak kernel / N, DC := 0,
- *                  times gain W(f/c) (f <= c, else 0), then / du.
- *   None: gain / N.
+ *   source 2 (SpatialRampFFT), any kind: taps ramp[0..m) centred on index 0
+ *                  (ramp[j] at (j - m/2) mod N) / N, DFT, extract; unless
+ *                  Ram-Lak, 0 above the cutoff and x gain W(f/c) (window_shape);
+ *                  DC := 0; x gain / du.
+ *   kind 10 (Custom): custom[k] x gain / N / du.
+ *   kind 0 (None): gain / N.
+ *   AnalyticFreq:  w = gain f W(f/c) / N / du for f = k/N <= c, else 0.
+ *   DiscreteRLFFT: extract of the DFT of the du=1 spatial Ram-Lak kernel / N,
+ *                  DC := 0, times gain W(f/c) (f <= c, else 0), then / du.
  */
-int yko_filter_weights(int kind, int source, double cutoff, double gain,
+int yko_filter_weights_ext(int kind, int source, int extract, double cutoff, double gain,
     double order, double beta, double tukey_alpha, int N, double du_mm,
-    double* w)
+    const double* ramp, int m, const double* custom, int nc_custom, double* w)
 {
     const int nc = N / 2 + 1;
     const double cc = cutoff > 0.0 ? cutoff : 0.5;
+    if (source == 2) {
+        if (!ramp || m <= 0 || !(m & 1) || m > N) return -1;
+        double* h = (double*)calloc((size_t)N, sizeof(double));
+        if (!h) return -2;
+        for (int j = 0; j < m; ++j) {
+            int idx = j - m / 2;
+            if (idx < 0) idx += N;
+            h[idx] = ramp[j] / (double)N;
+        }
+        yko_dft_extract(h, N, extract, w);
+        free(h);
+        if (kind != 1) {
+            for (int k = 0; k < nc; ++k) {
+                const double f = (double)k / (double)N;
+                w[k] = f > cc ? 0.0 : w[k] * gain * yko_window(kind, f / cc, order, beta, tukey_alpha, 0);
+            }
+        }
+        w[0] = 0.0;
+        for (int k = 0; k < nc; ++k) w[k] *= gain / du_mm;
+        return 0;
+    }
+    if (kind == 10) {
+        if (!custom || nc_custom != nc) return -1;
+        for (int k = 0; k < nc; ++k) w[k] = custom[k] * gain / (double)N / du_mm;
+        return 0;
+    }
     if (kind == 0) {
         for (int k = 0; k < nc; ++k) w[k] = gain / (double)N;
         return 0;
@@ -478,17 +526,16 @@ int yko_filter_weights(int kind, int source, double cutoff, double gain,
         return 0;
     }
     /* DiscreteRLFFT: h[0] = 1/4, h[odd n] = -1/(pi^2 n^2), wrapped for n > N/2 */
-    for (int k = 0; k < nc; ++k) {
-        double acc = 0.25;
-        for (int u = 1; u < N; ++u) {
-            const int n = u <= N / 2 ? u : u - N;
-            const int an = n < 0 ? -n : n;
-            if (!(an & 1)) continue;
-            const double h = -1.0 / (M_PI * M_PI * (double)n * (double)n);
-            acc += h * cos(2.0 * M_PI * (double)k * (double)u / (double)N);
-        }
-        w[k] = acc / (double)N;
+    double* h = (double*)calloc((size_t)N, sizeof(double));
+    if (!h) return -2;
+    h[0] = 0.25 / (double)N;
+    for (int u = 1; u < N; ++u) {
+        const int n = u <= N / 2 ? u : u - N;
+        const int an = n < 0 ? -n : n;
+        if (an & 1) h[u] = -1.0 / (M_PI * M_PI * (double)n * (double)n) / (double)N;
     }
+    yko_dft_extract(h, N, extract, w);
+    free(h);
     w[0] = 0.0;
     for (int k = 0; k < nc; ++k) {
         const double f = (double)k / (double)N;
@@ -497,6 +544,14 @@ int yko_filter_weights(int kind, int source, double cutoff, double gain,
     }
     for (int k = 0; k < nc; ++k) w[k] /= du_mm;
     return 0;
+}
+
+int yko_filter_weights(int kind, int source, double cutoff, double gain,
+    double order, double beta, double tukey_alpha, int N, double du_mm,
+    double* w)
+{
+    return yko_filter_weights_ext(kind, source, 0, cutoff, gain, order, beta, tukey_alpha, N, du_mm,
+        NULL, 0, NULL, 0, w);
 }
 
 /* h[n] = sum_{k=0}^{N-1} W[k] cos(2 pi k n / N), W[N-k] = w[k]: the impulse

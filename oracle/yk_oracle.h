@@ -93,6 +93,9 @@ double yko_window(int kind, double x, double order, double beta, double tukey_al
 int yko_filter_weights(int kind, int source, double cutoff, double gain,
     double order, double beta, double tukey_alpha, int padded_n, double du_mm,
     double* weights_out);
+int yko_filter_weights_ext(int kind, int source, int extract, double cutoff, double gain,
+    double order, double beta, double tukey_alpha, int N, double du_mm,
+    const double* ramp, int m, const double* custom, int nc_custom, double* w);
 void yko_filter_kernel_from_weights(const double* weights, int padded_n,
     double* h_out);
 /* method: 0 = direct circular convolution with h = IDFT(w), 1 = FP64 FFT */

@@ -27,7 +27,7 @@ D_OFFU, D_OFFV, D_DU, D_DV, D_RADIAL, D_NORMAL = 6, 7, 8, 9, 10, 13
 C_CU, C_CV, C_CD, C_CR, C_WBASE = 0, 4, 8, 12, 16
 
 FILTERS = {"none": 0, "ramlak": 1, "shepplogan": 2, "cosine": 3, "hann": 4,
-           "hamming": 5, "blackman": 6, "butterworth": 7, "kaiser": 8, "tukey": 9}
+           "hamming": 5, "blackman": 6, "butterworth": 7, "kaiser": 8, "tukey": 9, "custom": 10}
 
 
 class Volume(ctypes.Structure):
@@ -72,6 +72,8 @@ def lib():
             "yko_window": (c_double, [c_int, c_double, c_double, c_double, c_double, c_int]),
             "yko_filter_weights": (c_int, [c_int, c_int, c_double, c_double, c_double, c_double,
                                            c_double, c_int, c_double, dp]),
+            "yko_filter_weights_ext": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_double,
+                                               c_double, c_int, c_double, dp, c_int, dp, c_int, dp]),
             "yko_filter_kernel_from_weights": (None, [dp, c_int, dp]),
             "yko_filter_rows": (None, [dp, c_int, c_int, c_int, c_int, dp, ip, c_int, dp]),
             "yko_fft": (None, [dp, dp, c_int, c_int]),
@@ -186,10 +188,19 @@ def start_u(nu, n, offset_u_pix):
 
 
 def filter_weights(kind, n, du, cutoff=0.5, gain=1.0, order=2.0, beta=8.6, tukey_alpha=0.5,
-                   source=0) -> np.ndarray:
+                   source=0, extract=0, ramp=None, custom=None) -> np.ndarray:
+    """source 0 AnalyticFreq, 1 DiscreteRLFFT, 2 SpatialRampFFT (taps `ramp`);
+    kind "custom" takes `custom` (n/2 + 1 weights); extract 0 real part, 1 magnitude."""
     k = FILTERS[kind] if isinstance(kind, str) else int(kind)
     out = np.zeros(n // 2 + 1)
-    lib().yko_filter_weights(k, source, cutoff, gain, order, beta, tukey_alpha, n, du, _dp(out))
+    r = _d(ramp) if ramp is not None else None
+    c = _d(custom) if custom is not None else None
+    st = lib().yko_filter_weights_ext(k, source, extract, cutoff, gain, order, beta, tukey_alpha, n, du,
+                                      _dp(r) if r is not None else None, 0 if r is None else r.size,
+                                      _dp(c) if c is not None else None, 0 if c is None else c.size,
+                                      _dp(out))
+    if st != 0:
+        raise ValueError(f"oracle filter weights rejected the input ({st})")
     return out
 
 

@@ -119,7 +119,7 @@ def test_ideal_circle_closed_forms():
     np.testing.assert_allclose(d[:, 2], 1.0, rtol=1e-6)
 
 
-@pytest.mark.parametrize("kind", list(C.FILTERS))
+@pytest.mark.parametrize("kind", [k for k in C.FILTERS if k != "custom"])
 @pytest.mark.parametrize("n", [512, 2048])
 @pytest.mark.parametrize("cutoff", [0.5, 0.3])
 def test_filter_weights_match_oracle(oracle, kind, n, cutoff):
@@ -129,6 +129,113 @@ def test_filter_weights_match_oracle(oracle, kind, n, cutoff):
     f32 = lambda v: float(np.float32(v))  # noqa: E731
     o = oracle.filter_weights(kind, n, f32(0.8), cutoff=f32(cutoff), gain=f32(1.3))
     np.testing.assert_allclose(w, o, rtol=2e-6, atol=1e-12)
+
+
+def _rl_taps(m):
+    """the du = 1 sampled Ram-Lak kernel on taps -(m//2)..m//2"""
+    n = np.arange(m) - m // 2
+    t = np.zeros(m)
+    t[m // 2] = 0.25
+    odd = (n % 2) != 0
+    t[odd] = -1.0 / (np.pi ** 2 * n[odd].astype(np.float64) ** 2)
+    return t
+
+
+@pytest.mark.parametrize("kind", ["ramlak", "hann", "hamming", "shepplogan", "butterworth", "kaiser", "tukey"])
+@pytest.mark.parametrize("n", [512, 2048])
+@pytest.mark.parametrize("extract", ["real", "magnitude"])
+def test_discrete_rl_weights_match_oracle(oracle, kind, n, extract):
+    """DiscreteRLFFT source: the DFT of the sampled Ram-Lak kernel, windowed
+    by window_shape (Butterworth, Kaiser and Tukey give 1 there), as in
+    src/Filter/YkCreateFilterKernelFFT.hpp:192-257."""
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    w = yk.filter_weights(n, 0.8, kind=kind, cutoff=0.4, gain=1.3, source="discrete_rl", extract_mode=extract)
+    o = oracle.filter_weights(kind, n, f32(0.8), cutoff=f32(0.4), gain=f32(1.3), source=1,
+                              extract=C.RAMP_EXTRACT[extract])
+    np.testing.assert_allclose(w, o, rtol=2e-6, atol=1e-12 * np.abs(o).max())
+
+
+@pytest.mark.parametrize("kind", ["ramlak", "hann", "cosine", "kaiser"])
+@pytest.mark.parametrize("extract", ["real", "magnitude"])
+def test_spatial_ramp_weights_match_oracle(oracle, kind, extract):
+    """SpatialRampFFT source with caller taps (a smoothed, truncated ramp)."""
+    n = 1024
+    taps = (_rl_taps(63) * np.hanning(65)[1:-1]).astype(np.float32)
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    w = yk.filter_weights(n, 0.6, kind=kind, cutoff=0.45, gain=1.7, source="spatial_ramp",
+                          extract_mode=extract, spatial_ramp=taps)
+    o = oracle.filter_weights(kind, n, f32(0.6), cutoff=f32(0.45), gain=f32(1.7), source=2,
+                              extract=C.RAMP_EXTRACT[extract], ramp=taps.astype(np.float64))
+    np.testing.assert_allclose(w, o, rtol=2e-6, atol=1e-9 * np.abs(o).max())
+
+
+def test_custom_weights_match_oracle(oracle):
+    n = 512
+    c = np.linspace(0.0, 2.0, n // 2 + 1).astype(np.float32) ** 1.5
+    w = yk.filter_weights(n, 0.7, kind="custom", gain=1.25, custom_weights=c)
+    o = oracle.filter_weights("custom", n, float(np.float32(0.7)), gain=1.25, custom=c.astype(np.float64))
+    np.testing.assert_allclose(w, o, rtol=2e-6, atol=1e-12)
+
+
+def test_oracle_filter_sources_pinned(oracle):
+    """The oracle's three DFT-built paths against closed forms and each other:
+    - SpatialRamp holding the sampled Ram-Lak taps of |n| < N/2 is DiscreteRL
+      (the tap at N/2 is even, hence zero);
+    - the SpatialRamp DFT equals numpy's rfft of the N-periodic taps / N;
+    - windowed SpatialRamp carries the gain twice (the reference's order),
+      the Ram-Lak one once;
+    - for symmetric taps the magnitude is |real part|;
+    - Custom is the caller's weights x gain / N / du."""
+    n, du = 256, 0.5
+    taps = _rl_taps(n - 1)
+    sr = oracle.filter_weights("hann", n, du, cutoff=0.4, source=2, ramp=taps)
+    drl = oracle.filter_weights("hann", n, du, cutoff=0.4, source=1)
+    np.testing.assert_allclose(sr, drl, rtol=1e-12, atol=1e-15)
+    h = np.zeros(n)
+    h[(np.arange(taps.size) - taps.size // 2) % n] = taps
+    ref = np.fft.rfft(h).real / n
+    ref[0] = 0.0
+    got = oracle.filter_weights("ramlak", n, 1.0, source=2, ramp=taps)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-15)
+    g1 = oracle.filter_weights("hamming", n, du, source=2, ramp=taps, gain=1.0)
+    g3 = oracle.filter_weights("hamming", n, du, source=2, ramp=taps, gain=3.0)
+    np.testing.assert_allclose(g3, 9.0 * g1, rtol=1e-12)
+    r1 = oracle.filter_weights("ramlak", n, du, source=2, ramp=taps, gain=1.0)
+    r3 = oracle.filter_weights("ramlak", n, du, source=2, ramp=taps, gain=3.0)
+    np.testing.assert_allclose(r3, 3.0 * r1, rtol=1e-12)
+    mag = oracle.filter_weights("ramlak", n, du, source=2, ramp=taps, extract=1)
+    np.testing.assert_allclose(mag, np.abs(r1), rtol=1e-9, atol=1e-15)
+    c = np.arange(n // 2 + 1, dtype=np.float64)
+    np.testing.assert_allclose(oracle.filter_weights("custom", n, du, gain=2.0, custom=c), c * 2.0 / n / du,
+                               rtol=1e-15)
+
+
+def test_spatial_ramp_of_rl_taps_is_discrete_rl():
+    """The product's two DFT paths agree on the same kernel."""
+    n = 2048
+    taps = _rl_taps(n - 1).astype(np.float32)
+    a = yk.filter_weights(n, 0.3, kind="ramlak", source="spatial_ramp", spatial_ramp=taps)
+    b = yk.filter_weights(n, 0.3, kind="ramlak", source="discrete_rl")
+    np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-6 * np.abs(b).max())
+
+
+def test_filter_source_validation():
+    taps = _rl_taps(31).astype(np.float32)
+    bad = [
+        dict(kind="ramlak", source="spatial_ramp", spatial_ramp=taps[:-1]),          # even tap count
+        dict(kind="ramlak", source="spatial_ramp"),                                    # no taps
+        dict(kind="ramlak", source="spatial_ramp", spatial_ramp=np.r_[taps[:-1], 1.0]),  # not symmetric
+        dict(kind="ramlak", source="spatial_ramp", spatial_ramp=_rl_taps(1025).astype(np.float32)),  # > N
+        dict(kind="custom"),                                                           # no weights
+        dict(kind="custom", custom_weights=np.ones(100, np.float32)),                  # wrong count
+        dict(kind="custom", custom_weights=np.full(257, np.inf, np.float32)),          # not finite
+        dict(kind="ramlak", source=3),
+        dict(kind="ramlak", source="discrete_rl", extract_mode=2),
+    ]
+    for kw in bad:
+        with pytest.raises(yk.YkError) as e:
+            yk.filter_weights(512, 1.0, **kw)
+        assert e.value.status == C.INVALID_CONFIG, kw
 
 
 def test_padded_n():

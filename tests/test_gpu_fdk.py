@@ -12,7 +12,7 @@ from helpers import max_abs, oracle_volume, rel_rmse, seeded_projection, small_g
 pytestmark = pytest.mark.gpu
 
 
-def oracle_filtered(oracle, plan, geom, proj, filt="ramlak", cutoff=0.5, parker=None):
+def oracle_filtered(oracle, plan, geom, proj, filt="ramlak", cutoff=0.5, parker=None, **wkw):
     ov = oracle.views_from_float(plan.views)
     d = oracle.derive(ov, geom.nu, geom.nv, geom.scan_range)
     q = oracle.preweight(ov, d, proj.astype(np.float64))
@@ -20,7 +20,7 @@ def oracle_filtered(oracle, plan, geom, proj, filt="ramlak", cutoff=0.5, parker=
         q = oracle.parker(ov, d, q, parker[0], parker[1], geom.scan_range)
     n = oracle.padded_n(geom.nu)
     w = oracle.filter_weights(filt, n, float(np.float32(d[0, oracle.D_DU])),
-                              cutoff=float(np.float32(cutoff)))
+                              cutoff=float(np.float32(cutoff)), **wkw)
     su = [oracle.start_u(geom.nu, n, d[i, oracle.D_OFFU]) for i in range(len(ov))]
     return oracle.filter_rows(q, w, n, su, method=1), oracle.coeffs(ov, d)
 
@@ -38,6 +38,46 @@ def test_filter_stage_matches_oracle(ctx, oracle, case):
     ref, _ = oracle_filtered(oracle, plan, g, proj, filt=case["filt"])
     got = out.cpu().numpy()
     assert rel_rmse(got, ref) < 2e-6
+    assert max_abs(got, ref) < 1e-5 * np.abs(ref).max()
+
+
+def _rl_taps(m):
+    n = np.arange(m) - m // 2
+    t = np.zeros(m)
+    t[m // 2] = 0.25
+    odd = (n % 2) != 0
+    t[odd] = -1.0 / (np.pi ** 2 * n[odd].astype(np.float64) ** 2)
+    return t
+
+
+@pytest.mark.parametrize("source", ["discrete_rl", "spatial_ramp", "custom"])
+def test_filter_sources_stage_matches_oracle(ctx, oracle, source):
+    """The weight sources other than the analytic one through the filter
+    kernel (N = 1024): DiscreteRLFFT with a Hann window and magnitude
+    extraction, SpatialRampFFT with caller taps and a Hamming window (gain
+    1.2, applied twice as the reference does), Custom weights."""
+    g = small_geometry(channels=300, rows=5, channel_mm=0.1, row_mm=0.1, total_views=4, views_per_turn=24,
+                       volume=(24, 20, 18), voxel_mm=(0.05, 0.05, 0.05), offset_unv_mm=(0.15, 0.0, 0.0))
+    n = 1024
+    taps = (_rl_taps(41) * np.hanning(43)[1:-1]).astype(np.float32)
+    custom = (np.linspace(0.0, 0.5, n // 2 + 1) * np.cos(np.linspace(0.0, 1.4, n // 2 + 1))).astype(np.float32)
+    opts, wkw, filt = {
+        "discrete_rl": (dict(filter="hann", cutoff=0.45, filter_source="discrete_rl", extract_mode="magnitude"),
+                        dict(source=1, extract=1), "hann"),
+        "spatial_ramp": (dict(filter="hamming", gain=1.2, filter_source="spatial_ramp", spatial_ramp=taps),
+                         dict(source=2, ramp=taps.astype(np.float64), gain=float(np.float32(1.2))), "hamming"),
+        "custom": (dict(filter="custom", gain=0.9, custom_weights=custom),
+                   dict(custom=custom.astype(np.float64), gain=float(np.float32(0.9))), "custom"),
+    }[source]
+    cutoff = opts.get("cutoff", 0.5)
+    plan = yk.FdkPlan.from_geometry(ctx, g, parker="disabled", **opts)
+    assert plan.padded_n == n
+    proj = seeded_projection(5, g.total_views, g.nv, g.nu)
+    ref, _ = oracle_filtered(oracle, plan, g, proj, filt=filt, cutoff=cutoff, **wkw)
+    out, _ = plan.filter(torch.from_numpy(proj).cuda())
+    got = out.cpu().numpy()
+    print(f"filter source {source}: FP32 rel-RMSE {rel_rmse(got, ref):.2e}")
+    assert rel_rmse(got, ref) < 2e-6, rel_rmse(got, ref)
     assert max_abs(got, ref) < 1e-5 * np.abs(ref).max()
 
 
