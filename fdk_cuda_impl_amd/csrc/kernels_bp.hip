@@ -518,10 +518,10 @@ constexpr int kBatch = 8;  // LDS reads issued before their results are consumed
 
 // Geometry modes of the kernel: the regular circular detector; any other
 // detector with one reciprocal per sample; any other detector whose depth
-// denominator changes by at most kNewtonStep (relative) per slice, where one
-// Newton step from the previous sample's reciprocal, r (2 - den r), is exact
-// to FP32 (relative error below the square of that change) and takes two
-// full-rate instructions instead of the quarter-rate reciprocal.
+// denominator changes by at most 2e-4 (relative) per slice (fdk.cpp
+// general_mode), where u and v are quadratic in the slice to far below FP32
+// resolution of a pixel and are carried by forward differences (the mode's
+// name is that of its first form, one Newton step per sample).
 constexpr int kRegular = 0, kGeneralRcp = 1, kGeneralNewton = 2;
 
 // float LDS quads: float storage, and every storage on the general path
@@ -605,6 +605,44 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
         // this path is VALU-bound, and packed half lerps cost as many cycles
         // per FMA as FP32 ones on gfx950 plus a conversion per sample
         const int rowb = rowq * 16;
+        if constexpr (GEO == kGeneralNewton) {
+            // The depth denominator d0 + d1 k changes by at most 2e-4
+            // (relative) per slice on this path, so u(k) = (n0 + n1 k) /
+            // (d0 + d1 k) is quadratic in the slice k to far below 1e-4 pixel
+            // over a block, and is carried by forward differences of the
+            // tile-local coordinate: four additions per sample instead of a
+            // reciprocal step and three products.  The differences are the
+            // exact ones, not differences of rounded values (whose reciprocal
+            // rounding, ~1e-4 pixel at |u| ~ 1000, the recurrence would
+            // multiply by k and k^2):
+            //   du0 = u(1) - u(0) = (n1 - u(0) d1) / (d0 + d1),
+            //   d2u = du(1) - du(0) = -2 du0 d1 / (d0 + 2 d1).
+            // Over BZ slices the position stays within ~1e-4 pixel of the
+            // exact quotient, as close as the per-sample reciprocal.
+            const float r0 = __builtin_amdgcn_rcpf(den);
+            const float r1 = __builtin_amdgcn_rcpf(den + dd);
+            const float r2 = __builtin_amdgcn_rcpf(fmaf(2.f, dd, den));
+            const float ug = un * r0, vg = vn * r0;
+            float u = ug - ul0, v = vg - vl0;
+            float su = fmaf(-ug, dd, du) * r1, sv = fmaf(-vg, dd, dvn) * r1;
+            const float m2 = -2.f * dd * r2;
+            const float s2u = su * m2, s2v = sv * m2;
+            const char* base = static_cast<const char*>(lds);
+#pragma unroll
+            for (int k = 0; k < BZ; ++k) {
+                const float fu = __builtin_amdgcn_fractf(u), fv = __builtin_amdgcn_fractf(v);
+                const char* qp = base + __mul24(int(v), rowb) + int(u) * 16;
+                const float4 q = *reinterpret_cast<const float4*>(qp);
+                const float c0 = fmaf(fv, q.z, q.x);
+                const float c1 = fmaf(fv, q.w, q.y);
+                acc[k] = fmaf(w, fmaf(fu, c1 - c0, c0), acc[k]);
+                u += su;
+                v += sv;
+                su += s2u;
+                sv += s2v;
+            }
+            return;
+        }
         // fully unrolled: a partial unroll indexes acc[] through the register
         // index mode (a move in and out per sample).  (Issuing the LDS reads of
         // eight samples before consuming them measured slower: the extra VGPRs
@@ -627,7 +665,6 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
             den += dd;
             un += du;
             vn += dvn;
-            if constexpr (GEO == kGeneralNewton) r = r * fmaf(-den, r, 2.f);
         }
     }
 }
