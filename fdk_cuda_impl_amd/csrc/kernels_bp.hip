@@ -527,6 +527,19 @@ constexpr int kRegular = 0, kGeneralRcp = 1, kGeneralNewton = 2;
 // float LDS quads: float storage, and every storage on the general path
 constexpr bool bp_float_quads(int storage, int geo) { return storage == YK_F32 || geo != kRegular; }
 
+// byte offset of an LDS pointer, and a 16-byte LDS load at a byte offset
+typedef __attribute__((address_space(3))) const char lds_cchar;
+__device__ __forceinline__ unsigned lds_offset(const void* p)
+{
+    return unsigned(reinterpret_cast<uintptr_t>((lds_cchar*)p));
+}
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 lds_load_f4(unsigned offset)
+{
+    const f32x4 v = *(__attribute__((address_space(3))) const f32x4*)(uintptr_t)offset;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 template <int STORAGE, int BZ, int GEO>
 __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& c, short4 t,
     float X, float Y, float Z0, float vz, float (&acc)[BZ])
@@ -627,12 +640,17 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
             float su = fmaf(-ug, dd, du) * r1, sv = fmaf(-vg, dd, dvn) * r1;
             const float m2 = -2.f * dd * r2;
             const float s2u = su * m2, s2v = sv * m2;
-            const char* base = static_cast<const char*>(lds);
+            const unsigned base = lds_offset(lds);
 #pragma unroll
             for (int k = 0; k < BZ; ++k) {
                 const float fu = __builtin_amdgcn_fractf(u), fv = __builtin_amdgcn_fractf(v);
-                const char* qp = base + __mul24(int(v), rowb) + int(u) * 16;
-                const float4 q = *reinterpret_cast<const float4*>(qp);
+                // one lshl_add and one 24-bit mad (u, v >= 0 in the tile); the
+                // empty asm keeps the compiler from re-associating the sum
+                // into a multiply, a shift and a three-way add
+                unsigned ub = (unsigned(int(u)) << 4) + base;
+                asm volatile("" : "+v"(ub));
+                const unsigned qa = __umul24(unsigned(int(v)), unsigned(rowb)) + ub;
+                const float4 q = lds_load_f4(qa);
                 const float c0 = fmaf(fv, q.z, q.x);
                 const float c1 = fmaf(fv, q.w, q.y);
                 acc[k] = fmaf(w, fmaf(fu, c1 - c0, c0), acc[k]);
@@ -647,18 +665,17 @@ __device__ __forceinline__ void accumulate_view(const void* lds, const BpCoeff& 
         // index mode (a move in and out per sample).  (Issuing the LDS reads of
         // eight samples before consuming them measured slower: the extra VGPRs
         // cost more than the exposed latency.)
-        float r = __builtin_amdgcn_rcpf(den);
+        const unsigned base = lds_offset(lds);
 #pragma unroll
         for (int k = 0; k < BZ; ++k) {
-            if constexpr (GEO == kGeneralRcp) {
-                if (k > 0) r = __builtin_amdgcn_rcpf(den);
-            }
+            const float r = __builtin_amdgcn_rcpf(den);
             // u, v >= 1 inside the footprint: truncation is floor
             const float u = un * r - ul0;
             const float v = vn * r - vl0;
             const float fu = __builtin_amdgcn_fractf(u), fv = __builtin_amdgcn_fractf(v);
-            const char* qp = static_cast<const char*>(lds) + __mul24(int(v), rowb) + int(u) * 16;
-            const float4 q = *reinterpret_cast<const float4*>(qp);
+            unsigned ub = (unsigned(int(u)) << 4) + base;
+            asm volatile("" : "+v"(ub));
+            const float4 q = lds_load_f4(__umul24(unsigned(int(v)), unsigned(rowb)) + ub);
             const float c0 = fmaf(fv, q.z, q.x);
             const float c1 = fmaf(fv, q.w, q.y);
             acc[k] = fmaf(w, fmaf(fu, c1 - c0, c0), acc[k]);
