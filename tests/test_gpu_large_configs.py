@@ -45,6 +45,45 @@ def c4(ctx):
     P.close()
 
 
+@pytest.mark.parametrize("volume", ["phantom", "noise"])
+@pytest.mark.parametrize("kind", ["joseph", "siddon"])
+def test_c4_size_gradient_matches_oracle(ctx, oracle, kind, volume):
+    """grad 0.5 ||A x||^2 = A^T A x at the C4 volume and detector (256^3 @
+    0.5 mm, 512 x 384 @ 0.6 mm, SOD 500 / SDD 1000), on 8 views 45 degrees
+    apart so that the FP64 oracle finishes in seconds: FP32 autograd (the
+    separable forward and transpose kernels of the full-size scan) against the
+    oracle's FP64 projection and exact transpose, for the C4 workload's
+    ellipsoid phantom and for uniform white noise.  The bound is 5e-5, not the
+    1e-5 of the small-geometry test: the ray parameters are FP32 over the
+    ~1000 mm source-detector path (Siddon's t of a 0.5 mm voxel step, ~5e-4 at
+    t ~ 0.5, carries ~1e-4 relative rounding; the small test's 1 mm voxels on
+    a 240 mm path 8x less), which measured 1.4e-5 to 3.6e-5 here."""
+    gate = 5e-5
+    g = yk.ScanGeometry(8, 8, 500.0, 1000.0, 512, 384, 0.6, 0.6, volume=(256, 256, 256),
+                        voxel_mm=(0.5, 0.5, 0.5))
+    P = yk.Projector(ctx, g.views(), g.nu, g.nv, g.grid())
+    try:
+        if volume == "phantom":
+            ell = phantoms.random_ellipsoids(4, 64, semi_axes=(5.0, 30.0), density=(0.0, 0.05), center_radius=30.0)
+            x = ctx.phantom(ell, g.grid(), clamp=False).cpu().numpy()
+        else:
+            x = np.random.default_rng(7).uniform(0.0, 1.0, P.volume_shape).astype(np.float32)
+        xt = torch.from_numpy(x).cuda().requires_grad_(True)
+        r = ag.project(xt, P, kind)
+        (0.5 * torch.sum(r * r)).backward()
+        got = xt.grad.cpu().numpy()
+        ov = oracle.views_from_float(P.views)
+        vol = oracle.volume(*g.volume, *g.voxel_mm, center=g.center_mm)
+        fp = oracle.fp_joseph if kind == "joseph" else oracle.fp_siddon
+        bp = oracle.bp_joseph_adjoint if kind == "joseph" else oracle.bp_siddon_adjoint
+        ref = bp(ov, g.nu, g.nv, vol, fp(ov, g.nu, g.nv, vol, x))
+        err = rel_rmse(got, ref)
+        print(f"C4-size gradient ({kind}, {volume}, 8 views) vs FP64: rel-RMSE {err:.2e}")
+        assert err < gate, err
+    finally:
+        P.close()
+
+
 def test_c4_cgls_30_iterations(c4):
     g, P, truth, b, x, res = c4
     print(f"CGLS 30 at 256^3 x 360: ||r|| {res[0]:.4e} -> {res[-1]:.4e}, "
