@@ -7,7 +7,9 @@ IForwardOperator / IBackOperator (src/common/YkProjectionOperators.hpp:24-40).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -178,6 +180,30 @@ def _torch():
     return torch
 
 
+# Live library objects, closed by an exit hook registered with the first of
+# them (so it runs before exit hooks registered earlier, such as torch's),
+# plans and projectors before their contexts: their handles are then never
+# released by __del__ during interpreter teardown.
+_live = weakref.WeakSet()
+_exit_hook = False
+
+
+def _close_all():
+    for obj in sorted(list(_live), key=lambda o: isinstance(o, Context)):
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
+def _track(obj):
+    global _exit_hook
+    _live.add(obj)
+    if not _exit_hook:
+        atexit.register(_close_all)
+        _exit_hook = True
+
+
 class Context:
     """One HIP stream on one device (yk_ctx).  By default the context adopts
     PyTorch's current stream of that device so library work is ordered with the
@@ -186,6 +212,7 @@ class Context:
     def __init__(self, device: int = 0, use_torch_stream: bool = True, synchronous: bool = True):
         self.device = device
         self.handle = ctypes.c_void_p()
+        _track(self)
         s = _torch().cuda.current_stream(device) if use_torch_stream else None
         C.call("yk_ctx_create", device, ctypes.byref(self.handle))
         if s is not None:
@@ -309,6 +336,7 @@ class FdkPlan:
         self.ctx = ctx
         self.options = options or FdkOptions()
         self.handle = ctypes.c_void_p()
+        _track(self)
         desc = self.options.desc()
         if system is not None:
             C.call("yk_fdk_create_system", ctx.handle, ctypes.byref(desc), ctypes.byref(system),
@@ -481,6 +509,7 @@ class Projector:
         self.nu, self.nv, self.grid = nu, nv, grid
         self.n_views = self.views.shape[0]
         self.handle = ctypes.c_void_p()
+        _track(self)
         C.call("yk_projector_create", ctx.handle, _views_ptr(self.views), self.n_views, nu, nv,
                ctypes.byref(grid), scan_range, ctypes.byref(self.handle))
 
