@@ -987,6 +987,11 @@ siddon_fp_sep_kernel(const float* __restrict__ vzx, float* __restrict__ proj,
             tnz = vz_ok ? (b.lo[2] + float(iz + (sz > 0 ? 1 : 0)) * g.vz - src[2]) * rcz : 3.0e38f;
         }
         bool zin = live;  // lane still inside the volume in z
+        // The segment walked last: its voxel value is added when the next
+        // segment is reached, so each gather has a cell of the walk to land
+        // (the walk depends on no loaded value).  The sum keeps the segment
+        // order: the first step adds 0 x 0.
+        float pv = 0.f, pl = 0.f;
         float t = a0;
         const int max_steps = g.nx + g.ny + 2;
         for (int n = 0; n < max_steps && t < e0; ++n) {
@@ -997,7 +1002,13 @@ siddon_fp_sep_kernel(const float* __restrict__ vzx, float* __restrict__ proj,
             const float* col = vzx + ((size_t)ix * g.ny + iy) * g.nz;
             while (zin && la < lb) {
                 const float te = fminf(tnz, lb);
-                acc += col[iz] * (te - la);
+                acc += pv * pl;
+                // the load is issued after that sum (its offset passes
+                // through it), so the value needs no copy between registers
+                unsigned off = unsigned(iz);
+                asm volatile("" : "+v"(off) : "v"(acc));
+                pv = col[off];
+                pl = te - la;
                 la = te;
                 if (tnz <= lb) {
                     iz += sz;
@@ -1016,12 +1027,29 @@ siddon_fp_sep_kernel(const float* __restrict__ vzx, float* __restrict__ proj,
                 tny = (b.lo[1] + float(iy + (sy > 0 ? 1 : 0)) * g.vy - src[1]) * rcy;
             }
         }
+        acc += pv * pl;
     }
     if (!pix) return;
     const float len = sqrtf(ray[0] * ray[0] + ray[1] * ray[1] + ray[2] * ray[2]);
     const float out = live ? acc * len : 0.f;
     float* dst = proj + ((size_t)i * nv + v) * nu + u;
     *dst = accumulate ? *dst + out : out;
+}
+
+// t-interval of the walk inside cell c of one axis: the forward walk's own
+// plane crossings (plane - src) * (1 / ray); a ray parallel to the axis stays
+// in the cell its source coordinate lies in
+__device__ __forceinline__ void cell_interval(float s, float r, float lo, float vox, int c, int n,
+    float& ta, float& tb)
+{
+    if (fabsf(r) > 1e-8f) {
+        const float rc = 1.f / r;
+        const float t0 = (lo + float(c) * vox - s) * rc, t1 = (lo + float(c + 1) * vox - s) * rc;
+        ta = fmaxf(ta, fminf(t0, t1));
+        tb = fminf(tb, fmaxf(t0, t1));
+    } else if (min(n - 1, max(0, int(floorf((s - lo) / vox)))) != c) {
+        tb = -3.0e38f;
+    }
 }
 
 // Siddon transpose: one thread per voxel; for each view the candidate rays
@@ -1037,14 +1065,12 @@ siddon_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
     const int z = blockIdx.z;
     if (x >= g.nx || y >= g.ny) return;
     const Box b = volume_box(g);
-    const float vox[3] = { g.vx, g.vy, g.vz };
-    const int idx3[3] = { x, y, z };
-    float clo[3], chi[3];
-    for (int k = 0; k < 3; ++k) {
-        clo[k] = b.lo[k] + float(idx3[k]) * vox[k];
-        chi[k] = b.lo[k] + float(idx3[k] + 1) * vox[k];
-    }
+    float clo[3], chi[3];  // the voxel's cube
+    clo[0] = b.lo[0] + float(x) * g.vx; chi[0] = b.lo[0] + float(x + 1) * g.vx;
+    clo[1] = b.lo[1] + float(y) * g.vy; chi[1] = b.lo[1] + float(y + 1) * g.vy;
+    clo[2] = b.lo[2] + float(z) * g.vz; chi[2] = b.lo[2] + float(z + 1) * g.vz;
     const float margin = 0.05f;
+    const bool surface = x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1;
     float acc = 0.f;
     for (int i = 0; i < count; ++i) {
         const double* c = c64 + (size_t)i * 12;
@@ -1069,11 +1095,25 @@ siddon_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
         const float* pi = proj + (size_t)i * nu * nv;
         for (int vv = vlo; vv <= vhi; ++vv)
             for (int uu = ulo; uu <= uhi; ++uu) {
-                float src[3], ray[3], t0, t1, c0, c1;
+                float src[3], ray[3];
                 siddon_ray_geom(sv, uu, vv, src, ray);
-                if (!clip_ray(src, ray, b.lo, b.hi, t0, t1)) continue;
-                if (!clip_ray(src, ray, clo, chi, c0, c1)) continue;
-                const float a = fmaxf(t0, c0), e = fminf(t1, c1);
+                // the voxel's interval from the forward walk's own plane
+                // crossings; the volume clip (whose faces are a voxel or more
+                // away from an interior voxel's) binds only on the surface
+                float a = -3.0e38f, e = 3.0e38f;
+                cell_interval(src[0], ray[0], b.lo[0], g.vx, x, g.nx, a, e);
+                cell_interval(src[1], ray[1], b.lo[1], g.vy, y, g.ny, a, e);
+                cell_interval(src[2], ray[2], b.lo[2], g.vz, z, g.nz, a, e);
+                if (!(e > a)) continue;
+                if (surface) {
+                    float t0, t1;
+                    if (!clip_ray(src, ray, b.lo, b.hi, t0, t1)) continue;
+                    a = fmaxf(a, t0);
+                    e = fminf(e, t1);
+                } else {
+                    a = fmaxf(a, 0.f);
+                    e = fminf(e, 1.f);
+                }
                 if (!(e > a)) continue;
                 const float len = sqrtf(ray[0] * ray[0] + ray[1] * ray[1] + ray[2] * ray[2]);
                 acc += pi[(size_t)vv * nu + uu] * ((e - a) * len);
@@ -1121,22 +1161,6 @@ siddon_sep_table_kernel(const float* __restrict__ proj, const SiddonViewDev* __r
     for (int k = ty; k < 32; k += 8) {
         const int u = u0 + k, v = v0 + tx;
         if (u < nu && v < nvp) ray_wz[((size_t)i * nu + u) * nvp + v] = s_t[tx][k];
-    }
-}
-
-// t-interval of the walk inside cell c of one axis: the forward walk's own
-// plane crossings (plane - src) * (1 / ray); a ray parallel to the axis stays
-// in the cell its source coordinate lies in
-__device__ __forceinline__ void cell_interval(float s, float r, float lo, float vox, int c, int n,
-    float& ta, float& tb)
-{
-    if (fabsf(r) > 1e-8f) {
-        const float rc = 1.f / r;
-        const float t0 = (lo + float(c) * vox - s) * rc, t1 = (lo + float(c + 1) * vox - s) * rc;
-        ta = fmaxf(ta, fminf(t0, t1));
-        tb = fminf(tb, fmaxf(t0, t1));
-    } else if (min(n - 1, max(0, int(floorf((s - lo) / vox)))) != c) {
-        tb = -3.0e38f;
     }
 }
 
