@@ -633,26 +633,24 @@ joseph_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
         for (int vv = vlo; vv <= vhi; ++vv)
             for (int uu = ulo; uu <= uhi; ++uu) {
                 const size_t k = (size_t)vv * nu + uu;
+                // The candidate test is branch-free, so the three loads issue
+                // at once (a skip after the first load made them wait in
+                // turn); a rejected candidate leaves the sum as it is.
                 const float L = li[k];
-                if (L == 0.f) continue;
                 const float4 a = abi[k];
+                const float py = pi[k];
                 JRay r;
                 r.a1 = a.x; r.b1 = a.y; r.a2 = a.z; r.b2 = a.w; r.L = L;
                 float f1, f2;
                 joseph_sample(r, ax, s, f1, f2);
                 const float g1 = floorf(f1), g2 = floorf(f2);
-                if (g1 < -1.f || g2 < -1.f || g1 > float(ax.n1 - 1) || g2 > float(ax.n2 - 1)) continue;
-                const int j1 = int(g1), j2 = int(g2);
                 const float w1 = f1 - g1, w2 = f2 - g2;
-                float wa;
-                if (i1 == j1) wa = 1.f - w1;
-                else if (i1 == j1 + 1) wa = w1;
-                else continue;
-                float wb;
-                if (i2 == j2) wb = 1.f - w2;
-                else if (i2 == j2 + 1) wb = w2;
-                else continue;
-                acc += pi[k] * L * (wa * wb);
+                // the voxel is tap j or j + 1 of the ray's sample on each axis
+                const bool a0 = float(i1) == g1, a1 = float(i1) == g1 + 1.f;
+                const bool b0 = float(i2) == g2, b1 = float(i2) == g2 + 1.f;
+                const float wa = a0 ? 1.f - w1 : w1, wb = b0 ? 1.f - w2 : w2;
+                const float t = fmaf(py * L, wa * wb, acc);
+                acc = (L != 0.f && (a0 || a1) && (b0 || b1)) ? t : acc;
             }
     }
     float* dst = volume + ((size_t)z * g.ny + y) * g.nx + x;
@@ -1318,36 +1316,48 @@ fdk_operator_bp_kernel(const float* __restrict__ proj, float* __restrict__ volum
     float acc[kOpBpZ];
 #pragma unroll
     for (int k = 0; k < kOpBpZ; ++k) acc[k] = 0.f;
+    // reciprocal voxel sizes for the voxel-driven Joseph main-axis test
+    const float ivx = 1.f / g.vx, ivy = 1.f / g.vy, ivz = 1.f / g.vz;
     for (int i = 0; i < count; ++i) {
         const OpBpViewDev c = views[i];
         const float* p = proj + (size_t)i * nu * nv;
+        // the view's map at this voxel column; each slice adds its Z term
+        const float d0 = fmaf(c.cd[1], Y, fmaf(c.cd[0], X, c.cd[3]));
+        const float u0 = fmaf(c.cu[1], Y, fmaf(c.cu[0], X, c.cu[3]));
+        const float v0 = fmaf(c.cv[1], Y, fmaf(c.cv[0], X, c.cv[3]));
 #pragma unroll
         for (int k = 0; k < kOpBpZ; ++k) {
             if (z0 + k >= g.nz) break;
             const float Z = g.oz + (z0 + k) * g.vz;
-            const float den = c.cd[0] * X + c.cd[1] * Y + c.cd[2] * Z + c.cd[3];
+            const float den = fmaf(c.cd[2], Z, d0);
             if (!(den >= 1e-8f)) continue;
-            const float u = (c.cu[0] * X + c.cu[1] * Y + c.cu[2] * Z + c.cu[3]) / den;
-            const float v = (c.cv[0] * X + c.cv[1] * Y + c.cv[2] * Z + c.cv[3]) / den;
+            // one Newton step on the hardware reciprocal: u and v within an
+            // ulp or so of the quotients (the sample position is the
+            // sensitive input), for two FMAs instead of two divisions
+            float rd = __builtin_amdgcn_rcpf(den);
+            rd = fmaf(rd, fmaf(-den, rd, 1.f), rd);
+            const float u = fmaf(c.cu[2], Z, u0) * rd;
+            const float v = fmaf(c.cv[2], Z, v0) * rd;
             float w;
             if constexpr (MODE == kOpJosephVoxel) {
                 const float dx = X - c.src[0], dy = Y - c.src[1], dz = Z - c.src[2];
-                const float ax = fabsf(dx / g.vx), ay = fabsf(dy / g.vy), az = fabsf(dz / g.vz);
+                const float ax = fabsf(dx * ivx), ay = fabsf(dy * ivy), az = fabsf(dz * ivz);
                 const bool mx = ax >= ay && ax >= az, my = !mx && ay >= az;
                 const float vm = mx ? g.vx : my ? g.vy : g.vz;
                 const float dm = mx ? dx : my ? dy : dz;
-                w = vm * sqrtf(dx * dx + dy * dy + dz * dz) / fabsf(dm);
+                w = vm * sqrtf(dx * dx + dy * dy + dz * dz) * __builtin_amdgcn_rcpf(fabsf(dm));
             } else if constexpr (MODE == kOpMatched) {
                 const float L2 = c.l2[0] + u * (c.l2[1] + u * c.l2[3]) + v * (c.l2[2] + u * c.l2[4] + v * c.l2[5]);
                 if (!(L2 >= 1e-8f)) continue;
                 const float dx = X - c.src[0], dy = Y - c.src[1], dz = Z - c.src[2];
                 const float lsq = dx * dx + dy * dy + dz * dz;
                 if (!(lsq >= 1e-8f)) continue;
-                w = L2 * sqrtf(L2) / lsq * c.inv_sdd_plane * scale;
+                w = L2 * sqrtf(L2) * __builtin_amdgcn_rcpf(lsq) * c.inv_sdd_plane * scale;
             } else {
                 const float cn = c.cc[0] * X + c.cc[1] * Y + c.cc[2] * Z + c.cc[3];
                 if (!(fabsf(cn) >= 1e-8f)) continue;
-                w = c.wbase / (cn * cn);
+                const float rc = __builtin_amdgcn_rcpf(cn);
+                w = c.wbase * rc * rc;
             }
             acc[k] += bilinear_view(p, nu, nv, u, v) * w;
         }
