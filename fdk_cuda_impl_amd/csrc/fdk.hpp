@@ -99,8 +99,7 @@ struct yk_projector {
     bool any_x_axis = false;
     yk::DeviceArray<yk::ProjViewDev> d_jviews;
     yk::DeviceArray<yk::SiddonViewDev> d_sviews;
-    yk::DeviceArray<double> d_c64;
-    yk::DeviceArray<float> d_c32;      // the projective map in float (Joseph transpose candidates)
+    yk::DeviceArray<float> d_c32;      // the projective map in float (transpose candidates)
     yk::DeviceArray<float4> d_ray_ab;  // Joseph transpose ray table, one view chunk
     yk::DeviceArray<float> d_ray_len;
     std::vector<char> separable;       // per view: main axis X/Y and detV along Z only

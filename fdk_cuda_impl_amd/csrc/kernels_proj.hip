@@ -543,16 +543,6 @@ __global__ void transpose_xy_kernel(const float* __restrict__ in, float* __restr
     }
 }
 
-// projective map of a world point to detector pixel coordinates (FP64)
-__device__ __forceinline__ bool project64(const double* c, double X, double Y, double Z, double& u, double& v)
-{
-    const double den = c[8] * X + c[9] * Y + c[10] * Z + c[11];
-    if (!(den > 1e-9)) return false;
-    u = (c[0] * X + c[1] * Y + c[2] * Z + c[3]) / den;
-    v = (c[4] * X + c[5] * Y + c[6] * Z + c[7]) / den;
-    return true;
-}
-
 // Ray table of the Joseph transpose: the parameters of every ray of `count`
 // views, from the same joseph_ray routine the forward kernel calls (so the
 // transpose reuses bit-identical lines); L = 0 marks a ray the forward skips.
@@ -1055,7 +1045,7 @@ __device__ __forceinline__ void cell_interval(float s, float r, float lo, float 
 // contributes y * |segment inside the cube, clipped like the forward walk|.
 __global__ void __launch_bounds__(256)
 siddon_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume,
-    const SiddonViewDev* __restrict__ views, const double* __restrict__ c64, VolumeGeom g,
+    const SiddonViewDev* __restrict__ views, const float* __restrict__ c32, VolumeGeom g,
     int nu, int nv, int count, int accumulate)
 {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
@@ -1071,23 +1061,29 @@ siddon_adjoint_kernel(const float* __restrict__ proj, float* __restrict__ volume
     const bool surface = x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1;
     float acc = 0.f;
     for (int i = 0; i < count; ++i) {
-        const double* c = c64 + (size_t)i * 12;
-        double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+        // candidate pixels: the projected box of the voxel's 8 corners (FP32
+        // map; the 0.05 px margin covers its rounding, and every candidate
+        // is then tested with its own segment)
+        const float* c = c32 + (size_t)i * 12;
+        float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
         bool ok = true;
+#pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const double X = (k & 1) ? chi[0] : clo[0];
-            const double Y = (k & 2) ? chi[1] : clo[1];
-            const double Z = (k & 4) ? chi[2] : clo[2];
-            double uu, vv;
-            if (!project64(c, X, Y, Z, uu, vv)) { ok = false; break; }
-            umin = fmin(umin, uu); umax = fmax(umax, uu);
-            vmin = fmin(vmin, vv); vmax = fmax(vmax, vv);
+            const float X = (k & 1) ? chi[0] : clo[0];
+            const float Y = (k & 2) ? chi[1] : clo[1];
+            const float Z = (k & 4) ? chi[2] : clo[2];
+            const float den = c[8] * X + c[9] * Y + c[10] * Z + c[11];
+            ok = ok && den > 1e-6f;
+            const float rd = __builtin_amdgcn_rcpf(den);
+            const float uu = proj_coord(c, 0, X, Y, Z, rd), vv = proj_coord(c, 4, X, Y, Z, rd);
+            umin = fminf(umin, uu); umax = fmaxf(umax, uu);
+            vmin = fminf(vmin, vv); vmax = fmaxf(vmax, vv);
         }
         if (!ok) continue;
-        const int ulo = max(0, int(ceil(umin - margin)));
-        const int uhi = min(nu - 1, int(floor(umax + margin)));
-        const int vlo = max(0, int(ceil(vmin - margin)));
-        const int vhi = min(nv - 1, int(floor(vmax + margin)));
+        const int ulo = max(0, int(ceilf(umin - margin)));
+        const int uhi = min(nu - 1, int(floorf(umax + margin)));
+        const int vlo = max(0, int(ceilf(vmin - margin)));
+        const int vhi = min(nv - 1, int(floorf(vmax + margin)));
         if (ulo > uhi || vlo > vhi) continue;
         const SiddonViewDev sv = views[i];
         const float* pi = proj + (size_t)i * nu * nv;
@@ -1499,7 +1495,7 @@ void launch_siddon_adjoint_sep(const SiddonSepLaunch& p, hipStream_t s)
 void launch_siddon_adjoint(const SiddonAdjointLaunch& p, hipStream_t s)
 {
     dim3 block(256), grid((p.vol.nx + 15) / 16, (p.vol.ny + 15) / 16, p.vol.nz);
-    hipLaunchKernelGGL(siddon_adjoint_kernel, grid, block, 0, s, p.proj, p.volume, p.views, p.coeff64,
+    hipLaunchKernelGGL(siddon_adjoint_kernel, grid, block, 0, s, p.proj, p.volume, p.views, p.coeff32,
         p.vol, p.nu, p.nv, p.count, p.accumulate);
     YK_HIP(hipGetLastError());
 }

@@ -106,7 +106,6 @@ yk_projector::yk_projector(yk_ctx* c, const yk_view* v, int n, int nu_, int nv_,
     d_opviews.upload(ov.data(), ov.size(), s);
     d_jviews.upload(jv.data(), jv.size(), s);
     d_sviews.upload(sv.data(), sv.size(), s);
-    d_c64.upload(c64.data(), c64.size(), s);
     std::vector<float> c32(c64.begin(), c64.end());
     d_c32.upload(c32.data(), c32.size(), s);
     d_partials.resize(1024);
@@ -292,7 +291,7 @@ void yk_projector::backproject(int kind, const float* d_proj, float* d_volume, i
         p.proj = d_proj;
         p.volume = d_volume;
         p.views = d_sviews.get() + first;
-        p.coeff64 = d_c64.get() + size_t(first) * 12;
+        p.coeff32 = d_c32.get() + size_t(first) * 12;
         p.vol = vol;
         p.nu = nu; p.nv = nv; p.count = count;
         p.accumulate = accumulate;

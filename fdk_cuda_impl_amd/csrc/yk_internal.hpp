@@ -370,7 +370,7 @@ struct SiddonAdjointLaunch {
     const float* proj;
     float* volume;
     const SiddonViewDev* views;
-    const double* coeff64;
+    const float* coeff32;  // the projective map in float, 12 per view
     VolumeGeom vol;
     int nu, nv, count;
     int accumulate;
