@@ -469,6 +469,7 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
                 for (int r = 0; r < 8; ++r) bufb[slot<fft_out_pad(1)>(j * 8 + r)] = v[r];
             }
             __syncthreads();
+            __builtin_amdgcn_s_setprio(0);
             spec = fft_forward_ct<N, 8, true>(bufb, bufa, twr, wr, tw_lds);
         } else {
             // cosine (and Parker) weights on the way into LDS
@@ -488,6 +489,7 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
             }
             if constexpr (STORAGE != YK_F32) wave_max2_store(ma, mb, red);
             __syncthreads();
+            __builtin_amdgcn_s_setprio(0);
             if constexpr (NPT > 1) {
                 spec = fft_forward_ct<NC, 1, true>(bufa, bufb, twr, wr, tw_lds);
             } else {
@@ -523,6 +525,13 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
         // the next pair's texels load while this pair is stored (issued
         // here, not before the transforms, so that they hold no registers
         // through the passes and the row maxima)
+        // From here (the next pair's loads, this pair's crop and store, the
+        // barrier) through the next pair's weighting and first pass, the
+        // waves run at a raised priority; the transforms' LDS passes run at
+        // the normal one.  The scheduler then favours the workgroups between
+        // transforms over those inside one: -6 % for the FP16 output at
+        // n = 2048, -7 % for the FP32 output at n = 4096.
+        __builtin_amdgcn_s_setprio(3);
         if (p + int(gridDim.x) < pairs) prefetch(p + gridDim.x);
         const size_t base_a = (size_t)ra * nu, base_b = (size_t)rb * nu;
         for (int a = threadIdx.x; a < nu; a += kFilterThreads) {
@@ -536,6 +545,7 @@ fdk_filter_kernel(const float* __restrict__ proj, void* __restrict__ out,
         // the next pair overwrites bufa, which the crop (res) may be
         __syncthreads();
     }
+    __builtin_amdgcn_s_setprio(0);
 }
 
 template <int STORAGE, int NPT>
