@@ -113,6 +113,10 @@ def projector_leg(yk, phantoms, ctx, stream):
         out[f"{kind}_fp_ms"] = round(time_launches(lambda: P.forward(vol, kind=kind, out=proj), 3, 1, stream), 3)
         out[f"{kind}_adjoint_ms"] = round(time_launches(lambda: P.adjoint(proj, kind=kind, out=back), 3, 1,
                                                         stream), 3)
+    # the voxel-driven backprojections the iterative solvers use as B
+    for kind in ("fdk", "fdk_pseudo_matched", "joseph_voxel"):
+        out[f"{kind}_bp_ms"] = round(time_launches(lambda: P.adjoint(proj, kind=kind, out=back), 3, 1,
+                                                   stream), 3)
     P.close()
     # the general (non-separable) kernels: the same scan with the detector
     # tilted by (2, -1, 3) mrad about U, V, N
@@ -122,6 +126,8 @@ def projector_leg(yk, phantoms, ctx, stream):
     for kind in ("joseph", "siddon"):
         out[f"tilted_{kind}_fp_ms"] = round(time_launches(lambda: PT.forward(vol, kind=kind, out=proj), 3, 1,
                                                           stream), 3)
+        out[f"tilted_{kind}_adjoint_ms"] = round(time_launches(lambda: PT.adjoint(proj, kind=kind, out=back), 2, 1,
+                                                               stream), 3)
     PT.close()
     samples = 360 * geom.nu * geom.nv * 256
     out["joseph_fp_gsamples_per_s"] = round(samples / (out["joseph_fp_ms"] * 1e-3) / 1e9, 1)
