@@ -225,7 +225,7 @@ typedef float float4u __attribute__((ext_vector_type(4), aligned(8)));
 // positions are those of joseph_fp_kernel (the same IEEE operations) and the
 // ray sum keeps its slice order.
 template <int CW>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 joseph_fp_sep_kernel(const float* __restrict__ vzx, const float* __restrict__ vzy,
     float* __restrict__ proj, const ProjViewDev* __restrict__ views, VolumeGeom g,
     int nu, int nv, int count, int accumulate)
@@ -234,18 +234,19 @@ joseph_fp_sep_kernel(const float* __restrict__ vzx, const float* __restrict__ vz
     // offset of its tap row (the same in every lane: a1, b1 are exact
     // functions of the column u), built once per slice by one lane each and
     // read back per sample by a broadcast LDS read
-    __shared__ uint2 tab[4][64];
+    __shared__ uint2 tab[8][64];
     const int wave = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
-    // The four waves take CW neighbouring detector columns of 4 / CW
+    // The eight waves take CW neighbouring detector columns of 8 / CW
     // consecutive views: their rays nearly coincide, so the repeats of a tap
-    // row hit the CU's L1.  Column groups are remapped so that each XCD
+    // row hit the CU's L1 (eight views measured 8 % faster than four at
+    // 512^3 x 720, sixteen 7 % slower than eight).  Column groups are remapped so that each XCD
     // (workgroups go round-robin to the 8 XCDs; gridDim.x is a multiple of
     // 8) sweeps a contiguous range of columns, whose neighbouring rays share
     // tap rows in its L2.
     const int group = int(blockIdx.x & 7u) * int(gridDim.x >> 3) + int(blockIdx.x >> 3);
     const int u = group * CW + wave % CW;
     const int v = blockIdx.y * 64 + lane;
-    const int i = int(blockIdx.z) * (4 / CW) + wave / CW;
+    const int i = int(blockIdx.z) * (8 / CW) + wave / CW;
     if (u >= nu || i >= count) return;  // whole wave
     const bool pix = v < nv;
     const ProjViewDev pv = views[i];
@@ -334,17 +335,17 @@ joseph_fp_sep_kernel(const float* __restrict__ vzx, const float* __restrict__ vz
 // neighbouring columns.  As in joseph_fp_sep_kernel the sample position is
 // clamped to [-2, n], positions outside (-1, n) read only zeros, and a
 // sample is one 16-byte load and three lerps.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 joseph_fp_slab_kernel(const float* __restrict__ gx, const float* __restrict__ gy,
     float* __restrict__ proj, const ProjViewDev* __restrict__ views, VolumeGeom g,
     int nu, int nv, int count, int accumulate)
 {
-    // the four waves take one 64-ray row segment of four consecutive views
-    // (nearly coincident rays: the repeats of a sample's cache lines hit the
-    // CU's L1)
+    // the eight waves take one 64-ray row segment of eight consecutive
+    // views (nearly coincident rays: the repeats of a sample's cache lines
+    // hit the CU's L1; eight measured 7 % faster than four at 512^3 x 720)
     const int u = blockIdx.x * 64 + threadIdx.x;
     const int v = blockIdx.y;
-    const int i = blockIdx.z * 4 + threadIdx.y;
+    const int i = blockIdx.z * 8 + threadIdx.y;
     if (i >= count) return;  // whole wave
     const bool pix = u < nu && v < nv;
     const ProjViewDev pv = views[i];
@@ -1380,7 +1381,7 @@ void launch_joseph_fp_sep(const JosephSepFpLaunch& p, hipStream_t s)
 {
     constexpr int kCols = 1;  // columns per workgroup (the other waves: further views)
     const int groups = (p.nu + kCols - 1) / kCols;
-    dim3 block(256), grid((groups + 7) / 8 * 8, (p.nv + 63) / 64, (p.count + 4 / kCols - 1) / (4 / kCols));
+    dim3 block(512), grid((groups + 7) / 8 * 8, (p.nv + 63) / 64, (p.count + 8 / kCols - 1) / (8 / kCols));
     hipLaunchKernelGGL(joseph_fp_sep_kernel<kCols>, grid, block, 0, s, p.vol_zx, p.vol_zy, p.proj, p.views,
         p.vol, p.nu, p.nv, p.count, p.accumulate);
     YK_HIP(hipGetLastError());
@@ -1420,7 +1421,7 @@ void launch_joseph_pair_copy(const float* in, float* out, int nx, int ny, int nz
 
 void launch_joseph_fp_slab(const JosephSepFpLaunch& p, hipStream_t s)
 {
-    dim3 block(64, 4), grid((p.nu + 63) / 64, p.nv, (p.count + 3) / 4);
+    dim3 block(64, 8), grid((p.nu + 63) / 64, p.nv, (p.count + 7) / 8);
     hipLaunchKernelGGL(joseph_fp_slab_kernel, grid, block, 0, s, p.vol_zx, p.vol_zy, p.proj, p.views, p.vol,
         p.nu, p.nv, p.count, p.accumulate);
     YK_HIP(hipGetLastError());
