@@ -938,12 +938,16 @@ siddon_fp_kernel(const float* __restrict__ vol, float* __restrict__ proj,
 // [x][y][z] volume copy, where the lanes of a wave touch neighbouring z.
 __global__ void __launch_bounds__(256)
 siddon_fp_sep_kernel(const float* __restrict__ vzx, float* __restrict__ proj,
-    const SiddonViewDev* __restrict__ views, VolumeGeom g, int nu, int nv, int accumulate)
+    const SiddonViewDev* __restrict__ views, VolumeGeom g, int nu, int nv, int count, int accumulate)
 {
-    const int u = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    // the four waves take one column of four consecutive views: their rays
+    // nearly coincide, so the repeats of a voxel's cache line hit the CU's
+    // L1 (four views measured 3-4 % faster than four neighbouring columns of
+    // one view, eight views slightly slower than four)
+    const int u = blockIdx.x;
     const int v = blockIdx.y * 64 + int(threadIdx.x & 63);
-    const int i = blockIdx.z;
-    if (u >= nu) return;  // whole wave
+    const int i = int(blockIdx.z) * 4 + int(threadIdx.x >> 6);
+    if (i >= count) return;  // whole wave
     const bool pix = v < nv;
     const SiddonViewDev sv = views[i];
     float src[3], ray[3];
@@ -1438,9 +1442,9 @@ void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz,
 
 void launch_siddon_fp_sep(const SiddonSepFpLaunch& p, hipStream_t s)
 {
-    dim3 block(256), grid((p.nu + 3) / 4, (p.nv + 63) / 64, p.count);
+    dim3 block(256), grid(p.nu, (p.nv + 63) / 64, (p.count + 3) / 4);
     hipLaunchKernelGGL(siddon_fp_sep_kernel, grid, block, 0, s, p.vol_zx, p.proj, p.views, p.vol, p.nu, p.nv,
-        p.accumulate);
+        p.count, p.accumulate);
     YK_HIP(hipGetLastError());
 }
 
