@@ -506,25 +506,6 @@ __global__ void pair_slab_y_kernel(const float* __restrict__ in, float2* __restr
     out[(size_t(y) * rows + r) * cols + x + kJosephGuard] = make_float2(a, b);
 }
 
-// [z][y][x] -> Z-fastest copy: out[x * sx + y * sy + z], one y row per
-// blockIdx.z, 32 x 32 (x, z) tiles through LDS
-__global__ void transpose_zfast_kernel(const float* __restrict__ in, float* __restrict__ out,
-    int nx, int ny, int nz, size_t sx, size_t sy)
-{
-    __shared__ float tile[32][33];
-    const int y = blockIdx.z;
-    const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
-    for (int k = threadIdx.y; k < 32; k += 8) {
-        const int z = z0 + k, x = x0 + threadIdx.x;
-        if (x < nx && z < nz) tile[k][threadIdx.x] = in[((size_t)z * ny + y) * nx + x];
-    }
-    __syncthreads();
-    for (int k = threadIdx.y; k < 32; k += 8) {
-        const int x = x0 + k, z = z0 + threadIdx.x;
-        if (x < nx && z < nz) out[(size_t)x * sx + (size_t)y * sy + z] = tile[threadIdx.x][k];
-    }
-}
-
 // [z][y][x] -> [z][x][y]
 __global__ void transpose_xy_kernel(const float* __restrict__ in, float* __restrict__ out,
     int nx, int ny)
@@ -933,107 +914,6 @@ siddon_fp_kernel(const float* __restrict__ vol, float* __restrict__ proj,
     *dst = accumulate ? *dst + acc : acc;
 }
 
-// Siddon projection for separable views (detV along Z only).  One wave per
-// detector column (64 consecutive rays in v): the rays share ray_x and
-// ray_y, so the XY cells they cross and the XY plane crossings are the same
-// for every lane and are walked once per wave; each lane walks its own Z
-// crossings inside every cell.  The events (XY and Z plane crossings, clip
-// ends) are those of siddon_fp_kernel, computed by the same expressions, so
-// the segments and their order match that walk.  Reads a Z-fastest
-// [x][y][z] volume copy, where the lanes of a wave touch neighbouring z.
-__global__ void __launch_bounds__(256)
-siddon_fp_sep_kernel(const float* __restrict__ vzx, float* __restrict__ proj,
-    const SiddonViewDev* __restrict__ views, VolumeGeom g, int nu, int nv, int count, int accumulate)
-{
-    // the four waves take one column of four consecutive views: their rays
-    // nearly coincide, so the repeats of a voxel's cache line hit the CU's
-    // L1 (four views measured 3-4 % faster than four neighbouring columns of
-    // one view, eight views slightly slower than four)
-    const int u = blockIdx.x;
-    const int v = blockIdx.y * 64 + int(threadIdx.x & 63);
-    const int i = int(blockIdx.z) * 4 + int(threadIdx.x >> 6);
-    if (i >= count) return;  // whole wave
-    const bool pix = v < nv;
-    const SiddonViewDev sv = views[i];
-    float src[3], ray[3];
-    siddon_ray_geom(sv, u, pix ? v : 0, src, ray);
-    const Box b = volume_box(g);
-    // column part of the box clip (ray_x, ray_y are the same in every lane)
-    float a0 = 0.f, e0 = 1.f;
-    const bool colhit = slab_clip(src[0], ray[0], b.lo[0], b.hi[0], a0, e0) &&
-        slab_clip(src[1], ray[1], b.lo[1], b.hi[1], a0, e0);
-    float acc = 0.f;
-    float t0 = a0, t1 = e0;
-    const bool live = pix && colhit && slab_clip(src[2], ray[2], b.lo[2], b.hi[2], t0, t1) && t0 < t1;
-    if (colhit && a0 < e0) {
-        // XY walk state from the column's entry a0
-        int ix = min(g.nx - 1, max(0, int(floorf((src[0] + a0 * ray[0] - b.lo[0]) / g.vx))));
-        int iy = min(g.ny - 1, max(0, int(floorf((src[1] + a0 * ray[1] - b.lo[1]) / g.vy))));
-        const int sx = ray[0] >= 0.f ? 1 : -1, sy = ray[1] >= 0.f ? 1 : -1;
-        const bool vx_ok = fabsf(ray[0]) > 1e-8f, vy_ok = fabsf(ray[1]) > 1e-8f;
-        const float rcx = vx_ok ? 1.f / ray[0] : 0.f, rcy = vy_ok ? 1.f / ray[1] : 0.f;
-        float tnx = vx_ok ? (b.lo[0] + float(ix + (sx > 0 ? 1 : 0)) * g.vx - src[0]) * rcx : 3.0e38f;
-        float tny = vy_ok ? (b.lo[1] + float(iy + (sy > 0 ? 1 : 0)) * g.vy - src[1]) * rcy : 3.0e38f;
-        // Z walk state of this lane from its own entry t0
-        const int sz = ray[2] >= 0.f ? 1 : -1;
-        const bool vz_ok = fabsf(ray[2]) > 1e-8f;
-        const float rcz = vz_ok ? 1.f / ray[2] : 0.f;
-        int iz = 0;
-        float tnz = 3.0e38f;
-        if (live) {
-            iz = min(g.nz - 1, max(0, int(floorf((src[2] + t0 * ray[2] - b.lo[2]) / g.vz))));
-            tnz = vz_ok ? (b.lo[2] + float(iz + (sz > 0 ? 1 : 0)) * g.vz - src[2]) * rcz : 3.0e38f;
-        }
-        bool zin = live;  // lane still inside the volume in z
-        // The segment walked last: its voxel value is added when the next
-        // segment is reached, so each gather has a cell of the walk to land
-        // (the walk depends on no loaded value).  The sum keeps the segment
-        // order: the first step adds 0 x 0.
-        float pv = 0.f, pl = 0.f;
-        float t = a0;
-        const int max_steps = g.nx + g.ny + 2;
-        for (int n = 0; n < max_steps && t < e0; ++n) {
-            const float tc = fminf(fminf(tnx, tny), e0);  // end of the current XY cell
-            // this lane's part of the cell, split at its Z crossings
-            float la = fmaxf(t, t0);
-            const float lb = fminf(tc, t1);
-            const float* col = vzx + ((size_t)ix * g.ny + iy) * g.nz;
-            while (zin && la < lb) {
-                const float te = fminf(tnz, lb);
-                acc += pv * pl;
-                // the load is issued after that sum (its offset passes
-                // through it), so the value needs no copy between registers
-                unsigned off = unsigned(iz);
-                asm volatile("" : "+v"(off) : "v"(acc));
-                pv = col[off];
-                pl = te - la;
-                la = te;
-                if (tnz <= lb) {
-                    iz += sz;
-                    if (iz < 0 || iz >= g.nz) { zin = false; break; }
-                    tnz = (b.lo[2] + float(iz + (sz > 0 ? 1 : 0)) * g.vz - src[2]) * rcz;
-                }
-            }
-            t = tc;
-            if (tnx <= tny) {
-                ix += sx;
-                if (ix < 0 || ix >= g.nx) break;
-                tnx = (b.lo[0] + float(ix + (sx > 0 ? 1 : 0)) * g.vx - src[0]) * rcx;
-            } else {
-                iy += sy;
-                if (iy < 0 || iy >= g.ny) break;
-                tny = (b.lo[1] + float(iy + (sy > 0 ? 1 : 0)) * g.vy - src[1]) * rcy;
-            }
-        }
-        acc += pv * pl;
-    }
-    if (!pix) return;
-    const float len = sqrtf(ray[0] * ray[0] + ray[1] * ray[1] + ray[2] * ray[2]);
-    const float out = live ? acc * len : 0.f;
-    float* dst = proj + ((size_t)i * nv + v) * nu + u;
-    *dst = accumulate ? *dst + out : out;
-}
-
 // t-interval of the walk inside cell c of one axis: the forward walk's own
 // plane crossings (plane - src) * (1 / ray); a ray parallel to the axis stays
 // in the cell its source coordinate lies in
@@ -1433,23 +1313,6 @@ void launch_joseph_fp_slab(const JosephSepFpLaunch& p, hipStream_t s)
     dim3 block(64, 8), grid((p.nu + 63) / 64, p.nv, (p.count + 7) / 8);
     hipLaunchKernelGGL(joseph_fp_slab_kernel, grid, block, 0, s, p.vol_zx, p.vol_zy, p.proj, p.views, p.vol,
         p.nu, p.nv, p.count, p.accumulate);
-    YK_HIP(hipGetLastError());
-}
-
-void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz, bool x_major, hipStream_t s)
-{
-    dim3 block(32, 8), grid((nx + 31) / 32, (nz + 31) / 32, ny);
-    const size_t sx = x_major ? size_t(ny) * nz : size_t(nz);
-    const size_t sy = x_major ? size_t(nz) : size_t(nx) * nz;
-    hipLaunchKernelGGL(transpose_zfast_kernel, grid, block, 0, s, in, out, nx, ny, nz, sx, sy);
-    YK_HIP(hipGetLastError());
-}
-
-void launch_siddon_fp_sep(const SiddonSepFpLaunch& p, hipStream_t s)
-{
-    dim3 block(256), grid(p.nu, (p.nv + 63) / 64, (p.count + 3) / 4);
-    hipLaunchKernelGGL(siddon_fp_sep_kernel, grid, block, 0, s, p.vol_zx, p.proj, p.views, p.vol, p.nu, p.nv,
-        p.count, p.accumulate);
     YK_HIP(hipGetLastError());
 }
 

@@ -299,8 +299,6 @@ struct JosephSepFpLaunch {
     int accumulate;
 };
 void launch_joseph_fp_sep(const JosephSepFpLaunch& p, hipStream_t s);
-// [z][y][x] -> [x][y][z] (x_major) or [y][x][z]
-void launch_transpose_zfast(const float* in, float* out, int nx, int ny, int nz, bool x_major, hipStream_t s);
 // Joseph projection of X- or Y-main views of any orientation from paired
 // slab copies (vol_zx / vol_zy made by launch_joseph_pair_copy with slab)
 void launch_joseph_fp_slab(const JosephSepFpLaunch& p, hipStream_t s);
@@ -355,16 +353,6 @@ struct SiddonLaunch {
     int accumulate;
 };
 void launch_siddon_fp(const SiddonLaunch& p, hipStream_t s);
-// Siddon projection for separable views from the [x][y][z] volume copy
-struct SiddonSepFpLaunch {
-    const float* vol_zx;        // [x][y][z]
-    float* proj;                // [count][nv][nu]
-    const SiddonViewDev* views; // device, `count`
-    VolumeGeom vol;
-    int nu, nv, count;
-    int accumulate;
-};
-void launch_siddon_fp_sep(const SiddonSepFpLaunch& p, hipStream_t s);
 
 struct SiddonAdjointLaunch {
     const float* proj;
