@@ -859,6 +859,11 @@ __device__ __forceinline__ void siddon_ray_geom(const SiddonViewDev& sv, int u, 
     }
 }
 
+// Siddon projection of any view set: one lane per ray, an exact Amanatides-
+// Woo walk through the [z][y][x] volume, clipped to t in [0, 1] and to the
+// volume box.  Plane crossings are recomputed from integer indices as
+// (plane - src) * (1 / ray), the values the transposes rebuild, and the ray
+// sum adds the segments in walk order.
 __global__ void __launch_bounds__(256)
 siddon_fp_kernel(const float* __restrict__ vol, float* __restrict__ proj,
     const SiddonViewDev* __restrict__ views, VolumeGeom g, int nu, int nv, int count, int accumulate)
